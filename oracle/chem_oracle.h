@@ -1,0 +1,102 @@
+/*
+ * CPU oracle for the per-cell stiff chemistry advance -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain FP64 restatement of the algorithm behind PeleLM's
+ * ReactorBase::react(box, rY, rYsrc, T, rE, rEsrc, FC, mask, dt, time)
+ * (call site Source/PeleLM.cpp:5752-5758), used by tests/, by the smoke check and
+ * by the CPU-baseline leg of bench.py -- never by the product path.
+ *
+ *   - species thermo (NASA-7), net production rates, third-body / Lindemann / Troe
+ *     fall-off and equilibrium-constant reverse rates: Docs/source/manual/Model.rst:227-296;
+ *   - the constant-pressure reactor right-hand side with T re-solved from
+ *     (rho h)(t) = (rho h)_0 + F_h t on every call: Docs/source/manual/Model.rst:683-698,
+ *     per-cell N+1 layout Source/PeleLM_K.H:14-52, HY2T use Source/PeleLM_K.H:89,511-530;
+ *   - a variable-order, fixed-leading-coefficient BDF in Nordsieck form with Newton
+ *     (maxcor 3), dense partial-pivot LU of I - gamma J, WRMS error control and
+ *     CVODE's step/order heuristics (the solver the reference decks select with
+ *     cvode.solve_type = denseAJ_direct, cvode.max_order = 4,
+ *     Exec/RegTests/FlameSheet/inputs.3d-regt:84-90).  SUNDIALS is not vendored in the
+ *     reference (Submodules/sundials is empty), so this restates CVODE's published
+ *     algorithm (Hindmarsh et al., ACM TOMS 31(3), 2005; CVODE user guide, Mathematical
+ *     Considerations).
+ *
+ * Parity pins (tests/): thermo continuity and heats of formation, PMF far-field
+ * enthalpy balance and ideal-gas density, mass/element conservation of wdot,
+ * detailed balance at HP equilibrium, complex-step derivatives for the analytic
+ * Jacobian, mpmath golden vectors for wdot, closed-form linear ODEs for the BDF,
+ * and SciPy Radau fixtures at rtol 1e-13 for reactor advances.
+ *
+ * Layout: every multi-cell array is structure-of-arrays, component-major with the
+ * cell index fastest (x[k*ncell + c]), the flattened form of an amrex::Array4 box.
+ */
+#ifndef PELELM_CHEM_ORACLE_H
+#define PELELM_CHEM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_mech orc_mech;
+
+orc_mech* orc_load(const char* tbl_path);
+void orc_free(orc_mech* m);
+int orc_nspec(const orc_mech* m);
+int orc_nreac(const orc_mech* m);
+int orc_nelem(const orc_mech* m);
+void orc_molwt(const orc_mech* m, double* W);
+/* element composition, row-major [nspec][nelem] */
+void orc_elem_comp(const orc_mech* m, int* ecomp);
+double orc_gas_constant(const orc_mech* m);
+
+/* per-species cp/R, h/RT, s/R at T */
+void orc_thermo(const orc_mech* m, double T, double* cp_R, double* h_RT, double* s_R);
+/* net molar production rates [mol/cm^3/s] from concentrations C [mol/cm^3] and T [K];
+   qnet (nullable) receives the net rate of progress of every reaction */
+void orc_wdot(const orc_mech* m, const double* C, double T, double* wdot, double* qnet);
+/* forward and reverse rates of progress of every reaction [mol/cm^3/s] */
+void orc_rop(const orc_mech* m, const double* C, double T, double* qf, double* qr);
+/* T such that sum_k Y_k h_k(T) = h [erg/g]; Newton from Tguess.  *iters (nullable) */
+double orc_hy2t(const orc_mech* m, double h, const double* Y, double Tguess, int* iters);
+
+/* Reactor right-hand side of y = (rhoY_0..rhoY_{N-1}, T) at time t (CGS):
+   T is re-solved from (rhoH0 + fH t)/rho with y[N] as initial guess. */
+void orc_rhs(const orc_mech* m, double t, const double* y, const double* fY, double rhoH0, double fH,
+             double* ydot);
+/* Same right-hand side with T = y[N] taken as given (no HY2T). */
+void orc_rhs_fixedT(const orc_mech* m, const double* y, const double* fY, double fH, double* ydot);
+/* Analytic Jacobian d(orc_rhs_fixedT)/dy, row-major (N+1)x(N+1). */
+void orc_jac(const orc_mech* m, const double* y, const double* fY, double fH, double* J);
+/* Complex-step derivative of orc_rhs_fixedT (pins orc_jac), row-major. */
+void orc_jac_cs(const orc_mech* m, const double* y, const double* fY, double fH, double* J);
+
+typedef struct {
+  double rtol;
+  double atol;               /* scalar absolute tolerance ...                       */
+  const double* atol_vec;    /* ... or per-component (N+1) values, nullable          */
+  int max_order;             /* 1..5 */
+  long max_steps;
+} orc_opts;
+
+typedef struct {
+  long nst, nfe, nje, nsetups, nni, ncfn, netf;
+  int status;                /* 0 ok, <0 failure code */
+  int qlast;
+  double hlast;
+} orc_cellstats;
+
+/* Integrate every cell with mask >= 0 from 0 to dt (h-reactor, constant p).
+   Arrays are SoA component-major.  On return rY, T, rE hold the state at dt and
+   FC the number of RHS evaluations; mask < 0 cells are left untouched (FC = 0).
+   Returns the number of failed cells (0 on success). */
+int orc_react(const orc_mech* m, int ncell, double* rY, const double* rYsrc, double* T, double* rE,
+              const double* rEsrc, double* FC, const int* mask, double dt, const orc_opts* opts,
+              int nthreads, orc_cellstats* stats);
+
+/* Integrate the linear test system y' = A y (n x n, row-major) with the same BDF
+   machinery (pins the integrator against exp(A t)).  Returns status. */
+int orc_bdf_linear(int n, const double* A, double* y, double tout, const orc_opts* opts, orc_cellstats* st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
