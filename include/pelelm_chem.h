@@ -1,0 +1,126 @@
+/*
+ * pelelm_chem -- MI355X (gfx950) per-cell stiff chemistry reactor, C ABI.
+ *
+ * Drop-in for the call PeleLM makes through pele::physics::reactions::ReactorBase
+ * (Source/PeleLM.H:16,672-674): one reactor object per process, created from the
+ * `peleLM.chem_integrator` name, initialised as an h-reactor, given typical values for
+ * the absolute tolerances, asked to react boxes, closed at teardown.  Every entry point
+ * takes plain pointers and sizes; device arrays are HIP device pointers and all work is
+ * enqueued on the caller's HIP stream (asynchronous, like the reference's GPU path,
+ * which synchronises right after the call, Source/PeleLM.cpp:5773-5775).
+ *
+ * Units are CGS in and out, as the reference converts before and after the call
+ * (Source/PeleLM.cpp:5737-5746, 5764-5771).
+ *
+ * Return codes: 0 success, < 0 error (mi_chem_last_error describes it).
+ */
+#ifndef PELELM_CHEM_H
+#define PELELM_CHEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_OK 0
+#define MI_EINVAL -1      /* bad argument / unknown key or value            */
+#define MI_ESTATE -2      /* call order (react before init, ...)            */
+#define MI_EHIP -3        /* HIP runtime error                              */
+#define MI_ECELLFAIL -4   /* at least one cell failed to integrate          */
+#define MI_ENOMEM -5
+
+/* amrex::Array4<Real>-compatible view: element (i,j,k,n) lives at
+   p[(i-begin[0]) + (j-begin[1])*jstride + (k-begin[2])*kstride + n*nstride]. */
+typedef struct {
+  double* p;
+  int begin[3];
+  int end[3];           /* exclusive, as amrex::Array4::end */
+  long jstride, kstride, nstride;
+  int ncomp;
+} mi_real4;
+
+typedef struct {
+  int* p;
+  int begin[3];
+  int end[3];
+  long jstride, kstride, nstride;
+  int ncomp;
+} mi_int4;
+
+/* Per-call statistics (summed over cells of the last react call). */
+typedef struct {
+  long ncells;          /* cells integrated (mask >= 0)                         */
+  long nsteps, nfe, nje, nsetups, nni, ncfn, netf;
+  long nfail;           /* cells that did not reach dt                          */
+  long max_steps_cell;  /* largest step count of a single cell                  */
+  long nlu_refine;      /* iterative-refinement sweeps (mixed-precision LU)     */
+  long nlu_fallback;    /* FP32 factorisations replaced by FP64 ones            */
+  long npred_used;      /* Newton predictions taken from the learned predictor  */
+  long npred_fallback;  /* steps where the learned prediction was rejected      */
+  double ms_kernel;     /* device time of the integration kernel(s), ms         */
+  long ntrips;          /* right-hand-side trips the call took                  */
+} mi_chem_stats;
+
+/* ReactorBase::create(name) -- Source/PeleLM.cpp:706.
+   Accepted names: "ReactorCvode" / "ReactorBDF" (this reactor; dense analytic-Jacobian
+   BDF on the GPU) and "ReactorNull" (pass-through: rY += dt rYsrc, rE += dt rEsrc). */
+int mi_chem_create(const char* integrator, void** reactor);
+
+/* ReactorBase::init(reactor_type, ncells) -- Source/PeleLM.cpp:707-709.
+   reactor_type 2 = h-reactor (constant pressure, energy = rho h); 1 = e-reactor is
+   rejected.  ncells is a hint for workspace sizing. */
+int mi_chem_init(void* reactor, int reactor_type, int ncells);
+
+/* Runtime options, the ParmParse keys the reference decks use
+   (Exec/RegTests/FlameSheet/inputs.3d-regt:84-90) plus this reactor's own:
+     ode.rtol, ode.atol, cvode.max_order, cvode.max_num_steps (default 20000),
+     cvode.solve_type (denseAJ_direct), peleLM.use_typ_vals_chem (0/1),
+     mi.mechanism (drm19 | LiDryer), mi.schedule (queue: finished cells leave the work
+     lists, default | plain: fixed cell-to-lane mapping), mi.lu (fp64), mi.ir_sweeps,
+     mi.on_fail (continue | abort). */
+int mi_chem_set(void* reactor, const char* key, const char* value);
+
+/* ReactorBase::set_typ_vals_ode -- Source/PeleLM.cpp:1497-1516: n = nspec + 1 typical
+   values; with peleLM.use_typ_vals_chem = 1 the absolute tolerance of component i
+   becomes ode.atol * typ[i]. */
+int mi_chem_set_typ_vals(void* reactor, const double* typ, int n);
+
+/* ReactorBase::react(box, rY, rYsrc, T, rE, rEsrc, FC, mask, dt, time, stream) --
+   Source/PeleLM.cpp:5752-5758.  rY holds nspec components, rYsrc nspec, T/rE/rEsrc/FC
+   one each; cells of [lo, hi] (inclusive) with mask >= 0 are advanced from time to
+   time + dt in place.  On return *time += dt.  FC receives the number of RHS
+   evaluations per cell (negative for a failed cell). */
+int mi_chem_react_box(void* reactor, const int lo[3], const int hi[3], mi_real4 rY, mi_real4 rYsrc,
+                      mi_real4 T, mi_real4 rE, mi_real4 rEsrc, mi_real4 FC, mi_int4 mask, double* dt,
+                      double* time, void* hip_stream);
+
+/* Flat structure-of-arrays form of the same call: rY[k*ncells + c] (cell fastest),
+   rYsrc likewise (nullable = zero forcing), T/rE/FC[c], rEsrc[c] (nullable), mask[c]
+   (nullable = all cells active). */
+int mi_chem_react_flat(void* reactor, double* rY, const double* rYsrc, double* T, double* rE,
+                       const double* rEsrc, double* FC, const int* mask, int ncells, double* dt, double* time,
+                       void* hip_stream);
+
+/* Net production rates rho*wdot (CGS, g/cm^3/s) of every cell from (rhoY, rhoH, T):
+   T is first re-solved from h = rhoH/rho (Source/PeleLM_K.H:57-98 reactionRateRhoY, and
+   the T-from-(h,Y) update of Source/PeleLM_K.H:511-530).  SoA flat layout; T is updated
+   in place with the re-solved temperature; mask < 0 cells get zero rates. */
+int mi_chem_reaction_rates_flat(void* reactor, const double* rY, const double* rH, double* T,
+                                const int* mask, double* rYdot, int ncells, void* hip_stream);
+
+/* Statistics of the last react call (synchronises the stream of that call). */
+int mi_chem_get_stats(void* reactor, mi_chem_stats* out);
+
+/* Number of species of the selected mechanism, or < 0. */
+int mi_chem_nspec(void* reactor);
+
+const char* mi_chem_last_error(void* reactor);
+
+/* ReactorBase::close() -- Source/PeleLM.cpp:930.  Frees the reactor and its workspace. */
+int mi_chem_close(void* reactor);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,398 @@
+// C ABI of the MI355X reactor (include/pelelm_chem.h): reactor objects, the runtime
+// options the reference decks set through ParmParse, workspace management and the
+// launch of the integration kernels on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pelelm_chem.h"
+#include "reactor_internal.h"
+
+using namespace pelelm;
+
+namespace {
+
+struct Reactor {
+  std::string integrator;
+  bool null_reactor = false;
+  bool inited = false;
+  int reactor_type = 2;
+  int mech = MECH_DRM19;
+  // ode.* / cvode.* keys
+  double rtol = 1e-6;
+  double atol = 1e-10;
+  int max_order = 4;
+  long max_steps = 20000;
+  bool use_typ = false;
+  std::vector<double> typ;
+  // this reactor's keys
+  int schedule = SCHED_COMPACT;
+  int lu_mode = LU_FP64;
+  int ir_sweeps = 3;
+  bool on_fail_continue = true;
+  // device state
+  int device = -1;
+  int ncu = 0;
+  double* ws = nullptr;
+  int* iws = nullptr;
+  int* lists = nullptr;         // 4 lists of ws_cells ints
+  long ws_cells = 0;            // cells the workspace and lists hold
+  unsigned int* counts = nullptr;
+  unsigned long long* counters = nullptr;
+  long last_trips = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool have_stats = false;
+  long last_ncell = 0;
+  std::string err;
+};
+
+int fail(Reactor* r, int code, const std::string& msg) {
+  if (r) r->err = msg;
+  return code;
+}
+
+#define HIPCHK(r, call)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return fail((r), MI_EHIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+void free_workspace(Reactor* r) {
+  if (r->ws) (void)hipFree(r->ws);
+  if (r->iws) (void)hipFree(r->iws);
+  if (r->lists) (void)hipFree(r->lists);
+  r->ws = nullptr;
+  r->iws = nullptr;
+  r->lists = nullptr;
+  r->ws_cells = 0;
+}
+
+int ensure_device_state(Reactor* r, long ncell) {
+  if (r->device < 0) {
+    HIPCHK(r, hipGetDevice(&r->device));
+    HIPCHK(r, hipMalloc(&r->counts, 8 * sizeof(unsigned int)));
+    HIPCHK(r, hipMalloc(&r->counters, NCOUNTERS * sizeof(unsigned long long)));
+    HIPCHK(r, hipEventCreate(&r->ev0));
+    HIPCHK(r, hipEventCreate(&r->ev1));
+  }
+  if (ncell > r->ws_cells) {
+    free_workspace(r);
+    const long nw = (ncell + 63) / 64;
+    const size_t nd = (size_t)nw * ws_slots(r->mech) * 64;
+    const size_t ni = (size_t)nw * iws_slots(r->mech) * 64;
+    if (hipMalloc(&r->ws, nd * sizeof(double)) != hipSuccess)
+      return fail(r, MI_ENOMEM, "workspace allocation failed (" + std::to_string(nd * 8) + " bytes)");
+    if (hipMalloc(&r->iws, ni * sizeof(int)) != hipSuccess) return fail(r, MI_ENOMEM, "workspace allocation failed");
+    if (hipMalloc(&r->lists, 4 * (size_t)nw * 64 * sizeof(int)) != hipSuccess)
+      return fail(r, MI_ENOMEM, "list allocation failed");
+    r->ws_cells = nw * 64;
+  }
+  return MI_OK;
+}
+
+bool parse_double(const char* s, double* out) {
+  char* end = nullptr;
+  const double v = std::strtod(s, &end);
+  if (end == s || *end != 0 || !std::isfinite(v)) return false;
+  *out = v;
+  return true;
+}
+bool parse_long(const char* s, long* out) {
+  char* end = nullptr;
+  const long v = std::strtol(s, &end, 10);
+  if (end == s || *end != 0) return false;
+  *out = v;
+  return true;
+}
+
+int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream) {
+  if (!r->inited) return fail(r, MI_ESTATE, "react called before init");
+  if (!dt || !(*dt > 0.0)) return fail(r, MI_EINVAL, "dt must be positive");
+  if (!a.rY.p || !a.T.p || !a.rE.p || !a.FC.p) return fail(r, MI_EINVAL, "rY, T, rE and FC are required");
+  const int N = mech_nspec(r->mech);
+  const int N1 = N + 1;
+  if (N1 > NMAX1) return fail(r, MI_EINVAL, "mechanism too large");
+  a.dt = *dt;
+  a.rtol = r->rtol;
+  for (int i = 0; i < N1; ++i) a.atol[i] = r->use_typ && (int)r->typ.size() == N1 ? r->atol * r->typ[i] : r->atol;
+  a.qmax = r->max_order;
+  a.mxstep = r->max_steps;
+  a.lu_mode = r->lu_mode;
+  a.ir_sweeps = r->ir_sweeps;
+  a.compact = (r->schedule == SCHED_COMPACT) ? 1 : 0;
+  a.cell_status = nullptr;
+  r->last_ncell = a.ncell;
+  if (a.ncell == 0) {
+    *time += *dt;
+    return MI_OK;
+  }
+  int rc = ensure_device_state(r, r->null_reactor ? 0 : a.ncell);
+  if (rc) return rc;
+  a.ws = r->ws;
+  a.iws = r->iws;
+  for (int i = 0; i < 2; ++i) {
+    a.list[i] = r->lists + (size_t)i * r->ws_cells;
+    a.slist[i] = r->lists + (size_t)(2 + i) * r->ws_cells;
+  }
+  a.counts = r->counts;
+  a.counters = r->counters;
+  HIPCHK(r, launch_counters_reset(r->counters, stream));
+  HIPCHK(r, hipEventRecord(r->ev0, stream));
+  if (r->null_reactor) {
+    HIPCHK(r, launch_null(N, a, stream));
+  } else {
+    // every step takes at least one trip; 3 per step bounds retries generously
+    const long max_trips = 3 * r->max_steps + 100;
+    HIPCHK(r, run_react(r->mech, a, stream, max_trips, &r->last_trips));
+  }
+  HIPCHK(r, hipEventRecord(r->ev1, stream));
+  r->last_stream = stream;
+  r->have_stats = true;
+  *time += *dt;
+  return MI_OK;
+}
+
+ArrView flat_view(double* p, long ncell) {
+  ArrView v{};
+  v.p = p;
+  v.js = 0;
+  v.ks = 0;
+  v.ns = ncell;
+  return v;
+}
+
+ArrView box_view(const mi_real4& x, const int lo[3]) {
+  ArrView v{};
+  v.p = x.p;
+  v.js = x.jstride;
+  v.ks = x.kstride;
+  v.ns = x.nstride;
+  v.ox = lo[0] - x.begin[0];
+  v.oy = lo[1] - x.begin[1];
+  v.oz = lo[2] - x.begin[2];
+  return v;
+}
+
+bool box_covers(const mi_real4& x, const int lo[3], const int hi[3], int ncomp) {
+  if (!x.p) return true;
+  for (int d = 0; d < 3; ++d)
+    if (lo[d] < x.begin[d] || hi[d] >= x.end[d]) return false;
+  return x.ncomp >= ncomp;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mi_chem_create(const char* integrator, void** reactor) {
+  if (!reactor || !integrator) return MI_EINVAL;
+  *reactor = nullptr;
+  const std::string name(integrator);
+  auto* r = new Reactor();
+  r->integrator = name;
+  if (name == "ReactorNull") {
+    r->null_reactor = true;
+  } else if (name == "ReactorCvode" || name == "ReactorBDF") {
+    r->null_reactor = false;
+  } else {
+    delete r;
+    return MI_EINVAL;
+  }
+  *reactor = r;
+  return MI_OK;
+}
+
+int mi_chem_init(void* reactor, int reactor_type, int ncells) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (reactor_type != 2) return fail(r, MI_EINVAL, "only reactor_type 2 (h-reactor, constant pressure) is supported");
+  if (ncells < 0) return fail(r, MI_EINVAL, "ncells must be >= 0");
+  r->reactor_type = reactor_type;
+  r->inited = true;
+  return MI_OK;
+}
+
+int mi_chem_set(void* reactor, const char* key, const char* value) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (!key || !value) return fail(r, MI_EINVAL, "null key or value");
+  const std::string k(key), v(value);
+  double d;
+  long l;
+  if (k == "ode.rtol") {
+    if (!parse_double(value, &d) || d <= 0.0) return fail(r, MI_EINVAL, "ode.rtol must be a positive number");
+    r->rtol = d;
+  } else if (k == "ode.atol") {
+    if (!parse_double(value, &d) || d <= 0.0) return fail(r, MI_EINVAL, "ode.atol must be a positive number");
+    r->atol = d;
+  } else if (k == "cvode.max_order") {
+    if (!parse_long(value, &l) || l < 1 || l > 5) return fail(r, MI_EINVAL, "cvode.max_order must be 1..5");
+    r->max_order = (int)l;
+  } else if (k == "cvode.max_num_steps") {
+    if (!parse_long(value, &l) || l < 1) return fail(r, MI_EINVAL, "cvode.max_num_steps must be >= 1");
+    r->max_steps = l;
+  } else if (k == "cvode.solve_type") {
+    if (v != "denseAJ_direct" && v != "dense_direct")
+      return fail(r, MI_EINVAL, "cvode.solve_type: only the dense direct solve is implemented");
+  } else if (k == "peleLM.use_typ_vals_chem") {
+    if (!parse_long(value, &l) || (l != 0 && l != 1)) return fail(r, MI_EINVAL, "use_typ_vals_chem must be 0 or 1");
+    r->use_typ = (l == 1);
+  } else if (k == "mi.mechanism") {
+    const int m = mech_lookup(value);
+    if (m < 0) return fail(r, MI_EINVAL, "unknown mechanism " + v);
+    if (m != r->mech) {
+      r->mech = m;
+      free_workspace(r);
+      r->typ.clear();
+    }
+  } else if (k == "mi.schedule") {
+    if (v == "plain") r->schedule = SCHED_PLAIN;
+    else if (v == "queue") r->schedule = SCHED_COMPACT;
+    else return fail(r, MI_EINVAL, "mi.schedule must be plain or queue");
+  } else if (k == "mi.lu") {
+    if (v == "fp64") r->lu_mode = LU_FP64;
+    else return fail(r, MI_EINVAL, "mi.lu must be fp64");
+  } else if (k == "mi.ir_sweeps") {
+    if (!parse_long(value, &l) || l < 0 || l > 10) return fail(r, MI_EINVAL, "mi.ir_sweeps must be 0..10");
+    r->ir_sweeps = (int)l;
+  } else if (k == "mi.on_fail") {
+    if (v == "continue") r->on_fail_continue = true;
+    else if (v == "abort") r->on_fail_continue = false;
+    else return fail(r, MI_EINVAL, "mi.on_fail must be continue or abort");
+  } else {
+    return fail(r, MI_EINVAL, "unknown key " + k);
+  }
+  return MI_OK;
+}
+
+int mi_chem_set_typ_vals(void* reactor, const double* typ, int n) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (!typ || n != mech_nspec(r->mech) + 1) return fail(r, MI_EINVAL, "set_typ_vals expects nspec + 1 values");
+  for (int i = 0; i < n; ++i)
+    if (!(typ[i] > 0.0)) return fail(r, MI_EINVAL, "typical values must be positive");
+  r->typ.assign(typ, typ + n);
+  return MI_OK;
+}
+
+int mi_chem_react_flat(void* reactor, double* rY, const double* rYsrc, double* T, double* rE, const double* rEsrc,
+                       double* FC, const int* mask, int ncells, double* dt, double* time, void* hip_stream) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (ncells < 0) return fail(r, MI_EINVAL, "ncells must be >= 0");
+  if (!time) return fail(r, MI_EINVAL, "time is required");
+  ReactArgs a{};
+  a.rY = flat_view(rY, ncells);
+  a.rYsrc = flat_view(const_cast<double*>(rYsrc), ncells);
+  a.T = flat_view(T, ncells);
+  a.rE = flat_view(rE, ncells);
+  a.rEsrc = flat_view(const_cast<double*>(rEsrc), ncells);
+  a.FC = flat_view(FC, ncells);
+  a.mask.p = mask;
+  a.nx = std::max(ncells, 1);
+  a.ny = 1;
+  a.ncell = ncells;
+  return react_common(r, a, dt, time, static_cast<hipStream_t>(hip_stream));
+}
+
+int mi_chem_react_box(void* reactor, const int lo[3], const int hi[3], mi_real4 rY, mi_real4 rYsrc, mi_real4 T,
+                      mi_real4 rE, mi_real4 rEsrc, mi_real4 FC, mi_int4 mask, double* dt, double* time,
+                      void* hip_stream) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (!lo || !hi || !time) return fail(r, MI_EINVAL, "lo, hi and time are required");
+  for (int d = 0; d < 3; ++d)
+    if (hi[d] < lo[d]) return fail(r, MI_EINVAL, "empty box");
+  const int N = mech_nspec(r->mech);
+  if (!box_covers(rY, lo, hi, N) || !box_covers(rYsrc, lo, hi, N) || !box_covers(T, lo, hi, 1) ||
+      !box_covers(rE, lo, hi, 1) || !box_covers(rEsrc, lo, hi, 1) || !box_covers(FC, lo, hi, 1))
+    return fail(r, MI_EINVAL, "an array view does not cover the box or has too few components");
+  ReactArgs a{};
+  a.rY = box_view(rY, lo);
+  a.rYsrc = box_view(rYsrc, lo);
+  a.T = box_view(T, lo);
+  a.rE = box_view(rE, lo);
+  a.rEsrc = box_view(rEsrc, lo);
+  a.FC = box_view(FC, lo);
+  a.mask.p = mask.p;
+  a.mask.js = mask.jstride;
+  a.mask.ks = mask.kstride;
+  a.mask.ox = lo[0] - mask.begin[0];
+  a.mask.oy = lo[1] - mask.begin[1];
+  a.mask.oz = lo[2] - mask.begin[2];
+  a.nx = hi[0] - lo[0] + 1;
+  a.ny = hi[1] - lo[1] + 1;
+  a.ncell = (long)a.nx * a.ny * (hi[2] - lo[2] + 1);
+  return react_common(r, a, dt, time, static_cast<hipStream_t>(hip_stream));
+}
+
+int mi_chem_reaction_rates_flat(void* reactor, const double* rY, const double* rH, double* T, const int* mask,
+                                double* rYdot, int ncells, void* hip_stream) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (ncells < 0 || !rY || !rH || !T || !rYdot) return fail(r, MI_EINVAL, "bad arguments");
+  if (ncells == 0) return MI_OK;
+  HIPCHK(r, launch_rates(r->mech, rY, rH, T, mask, rYdot, ncells, static_cast<hipStream_t>(hip_stream)));
+  return MI_OK;
+}
+
+int mi_chem_get_stats(void* reactor, mi_chem_stats* out) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r || !out) return MI_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  if (!r->have_stats) return MI_OK;
+  HIPCHK(r, hipEventSynchronize(r->ev1));
+  unsigned long long c[NCOUNTERS];
+  HIPCHK(r, hipMemcpy(c, r->counters, sizeof(c), hipMemcpyDeviceToHost));
+  out->ncells = (long)c[CNT_CELLS];
+  out->nsteps = (long)c[CNT_NST];
+  out->nfe = (long)c[CNT_NFE];
+  out->nje = (long)c[CNT_NJE];
+  out->nsetups = (long)c[CNT_NSETUPS];
+  out->nni = (long)c[CNT_NNI];
+  out->ncfn = (long)c[CNT_NCFN];
+  out->netf = (long)c[CNT_NETF];
+  out->nfail = (long)c[CNT_NFAIL];
+  out->max_steps_cell = (long)c[CNT_MAXSTEPS];
+  out->nlu_refine = (long)c[CNT_IR];
+  out->nlu_fallback = (long)c[CNT_LUFALLBACK];
+  float ms = 0.f;
+  HIPCHK(r, hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  out->ms_kernel = ms;
+  out->ntrips = r->last_trips;
+  if (out->nfail > 0 && !r->on_fail_continue) return fail(r, MI_ECELLFAIL, "cells failed to integrate");
+  return MI_OK;
+}
+
+int mi_chem_nspec(void* reactor) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  return mech_nspec(r->mech);
+}
+
+const char* mi_chem_last_error(void* reactor) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return "null reactor";
+  return r->err.c_str();
+}
+
+int mi_chem_close(void* reactor) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  free_workspace(r);
+  if (r->counts) (void)hipFree(r->counts);
+  if (r->counters) (void)hipFree(r->counters);
+  if (r->ev0) (void)hipEventDestroy(r->ev0);
+  if (r->ev1) (void)hipEventDestroy(r->ev1);
+  delete r;
+  return MI_OK;
+}
+
+}  // extern "C"

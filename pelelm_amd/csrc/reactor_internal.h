@@ -1,0 +1,79 @@
+// Internal interface between the C ABI (chem_capi.cpp) and the HIP kernels
+// (reactor_bdf.hip).  Plain structs only: they are passed by value as kernel
+// arguments.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pelelm {
+
+constexpr int NMAX1 = 64;   // largest ODE size (species + 1) a kernel argument block carries
+
+// One strided device array over a box: element (cell, n) with cell -> (i, j, k),
+// i fastest, lives at p[(i + ox) + (j + oy) * js + (k + oz) * ks + n * ns].
+struct ArrView {
+  double* p;
+  long js, ks, ns;
+  int ox, oy, oz;
+};
+struct IntView {
+  const int* p;
+  long js, ks;
+  int ox, oy, oz;
+};
+
+enum Schedule : int { SCHED_PLAIN = 0, SCHED_COMPACT = 1 };
+enum LuMode : int { LU_FP64 = 0, LU_FP32_IR = 1 };
+
+// Per-call options shared by every schedule.
+struct ReactArgs {
+  ArrView rY, rYsrc, T, rE, rEsrc, FC;   // rYsrc.p / rEsrc.p may be null (zero forcing)
+  IntView mask;                          // mask.p may be null (all active)
+  int nx, ny;                            // box extents in i and j (cells are i-fastest)
+  long ncell;
+  double dt;
+  double rtol;
+  double atol[NMAX1];
+  int qmax;
+  long mxstep;
+  int lu_mode;
+  int ir_sweeps;
+  // per-cell workspace, tiled by 64 cells: element (slot s) of cell c at
+  // ws[((c / 64) * nslot + s) * 64 + c % 64]
+  double* ws;
+  int* iws;
+  // trip pipeline: active-cell lists (current / next), set-up lists and their counts
+  int compact;                           // 1: finished cells leave the lists
+  int* list[2];
+  int* slist[2];
+  unsigned int* counts;                  // 8 counters, see run_react
+  // outputs
+  unsigned long long* counters;          // per-call aggregate counters (see CounterIndex)
+  int* cell_status;                      // optional per-cell status (nullable)
+};
+
+enum CounterIndex : int {
+  CNT_CELLS = 0, CNT_NST, CNT_NFE, CNT_NJE, CNT_NSETUPS, CNT_NNI, CNT_NCFN, CNT_NETF, CNT_NFAIL,
+  CNT_MAXSTEPS, CNT_IR, CNT_LUFALLBACK, NCOUNTERS = 16
+};
+
+// Mechanism ids (order of the dispatch table in reactor_bdf.hip).
+enum MechId : int { MECH_LIDRYER = 0, MECH_DRM19 = 1, NMECH = 2 };
+
+int mech_nspec(int mech);
+const char* mech_name(int mech);
+int mech_lookup(const char* name);
+long ws_slots(int mech);        // doubles per lane
+long iws_slots(int mech);       // ints per lane
+
+// Integrate every cell described by `a`: trips of bounded kernels on `stream`, driven
+// from the host until no cell is active or max_trips trips have run (cells still
+// active then are reported failed).  Synchronises the stream once per trip.
+hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out);
+hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t stream);
+hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T, const int* mask, double* rYdot,
+                        long ncell, hipStream_t stream);
+hipError_t launch_null(int nspec, const ReactArgs& a, hipStream_t stream);
+
+}  // namespace pelelm

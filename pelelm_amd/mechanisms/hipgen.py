@@ -1,0 +1,363 @@
+"""Emit straight-line HIP device code for one mechanism (one cell per lane).
+
+The generated header defines ``struct mech_<name>`` with static members:
+
+* ``NS``, ``NR``, ``N1`` and the constant tables host code needs (molecular weights,
+  NASA-7 coefficients, species names);
+* ``thermo_cph(T, cpR, hRT)`` -- cp/R and h/RT of every species (HY2T, energy equation);
+* ``gibbs(T, logT, gRT)`` -- g/RT;
+* ``wdot(C, T, logT, invT, gRT, wd)`` -- net molar production rates [mol/cm^3/s];
+* ``jacobian(C, T, logT, invT, gRT, hRT, wd, dwdT, acc)`` -- the analytic derivatives
+  d(wdot_i W_i)/d(rhoY_k) accumulated through ``acc(i, k, value)`` plus dwdot/dT.
+
+NASA-7 range selection is branch-free (coefficients chosen with a select on
+T < Tmid) so lanes on both sides of the break temperature stay converged.  Constant
+folding: k_f with beta = Ea = 0 is a literal; Troe terms with T*** = 1e-30 / T* = 1e30
+vanish / become constants; third-body sums start from the total concentration and
+add (eff - 1) C_k only for species with non-unit efficiency.
+"""
+from __future__ import annotations
+
+import math
+
+from . import mechanism as M
+
+
+def _d(x: float) -> str:
+    r = repr(float(x))
+    if "e" not in r and "." not in r and "inf" not in r and "nan" not in r:
+        r += ".0"
+    return r
+
+
+def _cname(name: str) -> str:
+    return name.replace("(", "_").replace(")", "_").replace("-", "m").replace("*", "s")
+
+
+def _prod(terms: list) -> str:
+    return " * ".join(terms) if terms else "1.0"
+
+
+def _power_terms(side: dict) -> list:
+    t = []
+    for i, c in sorted(side.items()):
+        t += [f"C[{i}]"] * c
+    return t
+
+
+def _dpower_terms(side: dict, k: int) -> tuple[float, list]:
+    """d/dC_k of prod C^nu: coefficient and remaining factors."""
+    c = side[k]
+    t = []
+    for i, ci in sorted(side.items()):
+        n = ci - 1 if i == k else ci
+        t += [f"C[{i}]"] * n
+    return float(c), t
+
+
+def _thermo_groups(mech: M.Mechanism) -> dict:
+    groups: dict = {}
+    for k, s in enumerate(mech.species):
+        groups.setdefault(s.T[1], []).append(k)
+    return groups
+
+
+def gen_thermo(mech: M.Mechanism) -> str:
+    out = []
+    out.append("static __device__ __forceinline__ void thermo_cph(const double T, double* __restrict__ cpR, "
+               "double* __restrict__ hRT) {")
+    out.append("  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, iT = 1.0 / T;")
+    for tmid, ks in _thermo_groups(mech).items():
+        out.append(f"  {{ const bool lo = T < {_d(tmid)};")
+        for k in ks:
+            s = mech.species[k]
+            a = [f"(lo ? {_d(s.low[i])} : {_d(s.high[i])})" for i in range(7)]
+            out.append(f"    {{ const double a0 = {a[0]}, a1 = {a[1]}, a2 = {a[2]}, a3 = {a[3]}, a4 = {a[4]}, a5 = {a[5]};")
+            out.append(f"      cpR[{k}] = a0 + a1 * T + a2 * T2 + a3 * T3 + a4 * T4;")
+            out.append(f"      hRT[{k}] = a0 + 0.5 * a1 * T + (1.0 / 3.0) * a2 * T2 + 0.25 * a3 * T3 + 0.2 * a4 * T4 + a5 * iT; }}")
+        out.append("  }")
+    out.append("}")
+    out.append("")
+    out.append("static __device__ __forceinline__ void thermo_dcp(const double T, double* __restrict__ dcpR) {")
+    out.append("  const double T2 = T * T, T3 = T2 * T;")
+    for tmid, ks in _thermo_groups(mech).items():
+        out.append(f"  {{ const bool lo = T < {_d(tmid)};")
+        for k in ks:
+            s_ = mech.species[k]
+            a = [f"(lo ? {_d(s_.low[i])} : {_d(s_.high[i])})" for i in range(5)]
+            out.append(f"    dcpR[{k}] = {a[1]} + 2.0 * {a[2]} * T + 3.0 * {a[3]} * T2 + 4.0 * {a[4]} * T3;")
+        out.append("  }")
+    out.append("}")
+    out.append("")
+    out.append("static __device__ __forceinline__ void gibbs(const double T, const double logT, double* __restrict__ gRT) {")
+    out.append("  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, iT = 1.0 / T;")
+    for tmid, ks in _thermo_groups(mech).items():
+        out.append(f"  {{ const bool lo = T < {_d(tmid)};")
+        for k in ks:
+            s = mech.species[k]
+            a = [f"(lo ? {_d(s.low[i])} : {_d(s.high[i])})" for i in range(7)]
+            out.append(f"    {{ const double a0 = {a[0]}, a1 = {a[1]}, a2 = {a[2]}, a3 = {a[3]}, a4 = {a[4]}, a5 = {a[5]}, a6 = {a[6]};")
+            # g/RT = h/RT - s/R
+            out.append(f"      gRT[{k}] = a0 * (1.0 - logT) - 0.5 * a1 * T - (1.0 / 6.0) * a2 * T2 - (1.0 / 12.0) * a3 * T3"
+                       f" - 0.05 * a4 * T4 + a5 * iT - a6; }}")
+        out.append("  }")
+    out.append("}")
+    return "\n".join(out)
+
+
+def _kf_expr(A: float, b: float, Ta: float) -> str:
+    if b == 0.0 and Ta == 0.0:
+        return _d(A)
+    parts = [_d(math.log(A))]
+    if b != 0.0:
+        parts.append(f"{_d(b)} * logT")
+    if Ta != 0.0:
+        parts.append(f"{_d(-Ta)} * invT")
+    return f"exp({' + '.join(parts)})"
+
+
+def _dlnk_expr(b: float, Ta: float) -> str:
+    # d ln(A T^b exp(-Ta/T)) / dT = (b + Ta/T) / T
+    if b == 0.0 and Ta == 0.0:
+        return "0.0"
+    if Ta == 0.0:
+        return f"{_d(b)} * invT"
+    if b == 0.0:
+        return f"{_d(Ta)} * invT * invT"
+    return f"({_d(b)} + {_d(Ta)} * invT) * invT"
+
+
+def _mc_expr(r: M.Reaction, ns: int) -> str:
+    terms = ["Ctot"]
+    for k in range(ns):
+        e = r.efficiencies.get(k, 1.0)
+        if e != 1.0:
+            terms.append(f"{_d(e - 1.0)} * C[{k}]")
+    return " + ".join(terms)
+
+
+def _troe_fc(r: M.Reaction) -> tuple[str, str]:
+    """Fcent and dFcent/dT expressions with constant folding of the 1e-30 / 1e30 forms."""
+    a = r.troe[0]
+    T3, T1 = r.troe[1], r.troe[2]
+    fc, dfc = [], []
+    if (1.0 - a) != 0.0 and T3 > 1e-20:
+        fc.append(f"{_d(1.0 - a)} * exp({_d(-1.0 / T3)} * T)")
+        dfc.append(f"{_d(-(1.0 - a) / T3)} * exp({_d(-1.0 / T3)} * T)")
+    if a != 0.0:
+        if T1 >= 1e20:
+            fc.append(_d(a))
+        else:
+            fc.append(f"{_d(a)} * exp({_d(-1.0 / T1)} * T)")
+            dfc.append(f"{_d(-a / T1)} * exp({_d(-1.0 / T1)} * T)")
+    if len(r.troe) == 4:
+        T2 = r.troe[3]
+        fc.append(f"exp({_d(-T2)} * invT)")
+        dfc.append(f"{_d(T2)} * invT * invT * exp({_d(-T2)} * invT)")
+    return (" + ".join(fc) or "0.0"), (" + ".join(dfc) or "0.0")
+
+
+def _kc_exprs(r: M.Reaction) -> tuple[str, str, int]:
+    """Sum of nu g/RT (products minus reactants), the same for h/RT, and sum nu."""
+    def side(arr: str) -> str:
+        out = ""
+        for i, c in sorted(r.products.items()):
+            t = f"{_d(c)} * {arr}[{i}]" if c != 1 else f"{arr}[{i}]"
+            out += t if not out else f" + {t}"
+        for i, c in sorted(r.reactants.items()):
+            t = f"{_d(c)} * {arr}[{i}]" if c != 1 else f"{arr}[{i}]"
+            out += f" - {t}"
+        return out
+    dnu = sum(r.products.values()) - sum(r.reactants.values())
+    return side("gRT"), side("hRT"), dnu
+
+
+def _refc_factor(dnu: int) -> str:
+    # (P_atm / RT)^(-dnu) = (RT/P_atm)^dnu = refCinv^dnu
+    if dnu == 0:
+        return ""
+    if dnu > 0:
+        return " * " + " * ".join(["refCinv"] * dnu)
+    return " * " + " * ".join(["refC"] * (-dnu))
+
+
+def gen_wdot(mech: M.Mechanism) -> str:
+    ns = mech.nspec
+    out = []
+    out.append("static __device__ __forceinline__ void wdot(const double* __restrict__ C, const double T, const double logT, "
+               "const double invT, const double* __restrict__ gRT, double* __restrict__ wd) {")
+    out.append(f"  for (int k = 0; k < {ns}; ++k) wd[k] = 0.0;")
+    out.append(f"  const double refC = {_d(M.P_ATM / M.R_CGS)} * invT, refCinv = 1.0 / refC;")
+    if any(r.has_third_body for r in mech.reactions):
+        out.append("  double Ctot = 0.0;")
+        out.append(f"  for (int k = 0; k < {ns}; ++k) Ctot += C[k];")
+    for j, r in enumerate(mech.reactions):
+        out.append(f"  {{ // {j}: {r.equation}")
+        out.append(f"    const double kf = {_kf_expr(r.A, r.beta, r.Ea / M.R_CAL)};")
+        if r.kind == "elementary":
+            out.append("    const double keff = kf;")
+        elif r.kind == "three-body":
+            out.append(f"    const double keff = kf * ({_mc_expr(r, ns)});")
+        else:
+            A0, b0, E0 = r.low
+            out.append(f"    const double k0 = {_kf_expr(A0, b0, E0 / M.R_CAL)};")
+            out.append(f"    const double Pr = k0 * ({_mc_expr(r, ns)}) / kf;")
+            if r.troe is not None:
+                fc, _ = _troe_fc(r)
+                out.append(f"    const double logFc = log10(fmax({fc}, 1e-300));")
+                out.append("    const double cc = -0.4 - 0.67 * logFc, nn = 0.75 - 1.27 * logFc;")
+                out.append("    const double u = log10(fmax(Pr, 1e-300)) + cc;")
+                out.append("    const double f1 = u / (nn - 0.14 * u);")
+                out.append("    const double F = exp(2.302585092994045684 * logFc / (1.0 + f1 * f1));")
+                out.append("    const double keff = kf * Pr / (1.0 + Pr) * F;")
+            else:
+                out.append("    const double keff = kf * Pr / (1.0 + Pr);")
+        pf = _prod(_power_terms(r.reactants))
+        if r.reversible:
+            gs, _, dnu = _kc_exprs(r)
+            pr = _prod(_power_terms(r.products))
+            out.append(f"    const double q = keff * ({pf} - {pr} * exp({gs}){_refc_factor(dnu)});")
+        else:
+            out.append(f"    const double q = keff * ({pf});")
+        for i, c in sorted(r.reactants.items()):
+            out.append(f"    wd[{i}] -= {_d(c) + ' * ' if c != 1 else ''}q;")
+        for i, c in sorted(r.products.items()):
+            out.append(f"    wd[{i}] += {_d(c) + ' * ' if c != 1 else ''}q;")
+        out.append("  }")
+    out.append("}")
+    return "\n".join(out)
+
+
+def gen_jacobian(mech: M.Mechanism) -> str:
+    """Derivatives of f_i = W_i wdot_i with respect to rhoY_k (=W_k C_k) and T.
+
+    acc(i, k, v) receives v = W_i/W_k * nu_ij * dq_j/dC_k contributions; dwdT[i] the
+    molar dwdot_i/dT; wd the molar production rates at the same state.
+    """
+    ns = mech.nspec
+    W = mech.W
+    out = []
+    out.append("template <class Acc>")
+    out.append("static __device__ __forceinline__ void jacobian(const double* __restrict__ C, const double T, const double logT, "
+               "const double invT, const double* __restrict__ gRT, const double* __restrict__ hRT, "
+               "double* __restrict__ wd, double* __restrict__ dwdT, Acc&& acc) {")
+    out.append(f"  for (int k = 0; k < {ns}; ++k) {{ wd[k] = 0.0; dwdT[k] = 0.0; }}")
+    out.append(f"  const double refC = {_d(M.P_ATM / M.R_CGS)} * invT, refCinv = 1.0 / refC;")
+    if any(r.has_third_body for r in mech.reactions):
+        out.append("  double Ctot = 0.0;")
+        out.append(f"  for (int k = 0; k < {ns}; ++k) Ctot += C[k];")
+    for j, r in enumerate(mech.reactions):
+        out.append(f"  {{ // {j}: {r.equation}")
+        out.append(f"    const double kf = {_kf_expr(r.A, r.beta, r.Ea / M.R_CAL)};")
+        out.append(f"    const double dlnkf = {_dlnk_expr(r.beta, r.Ea / M.R_CAL)};")
+        if r.kind == "elementary":
+            out.append("    const double keff = kf, dkeffdT = kf * dlnkf;")
+        elif r.kind == "three-body":
+            out.append(f"    const double Mc = {_mc_expr(r, ns)};")
+            out.append("    const double keff = kf * Mc, dkeffdT = keff * dlnkf, dkeffdM = kf;")
+        else:
+            A0, b0, E0 = r.low
+            out.append(f"    const double Mc = {_mc_expr(r, ns)};")
+            out.append(f"    const double k0 = {_kf_expr(A0, b0, E0 / M.R_CAL)};")
+            out.append(f"    const double dlnk0 = {_dlnk_expr(b0, E0 / M.R_CAL)};")
+            out.append("    const double Pr = k0 * Mc / kf, dPrdM = k0 / kf, dPrdT = Pr * (dlnk0 - dlnkf);")
+            if r.troe is not None:
+                fc, dfc = _troe_fc(r)
+                out.append(f"    double Fc = {fc}, dFc = {dfc};")
+                out.append("    if (Fc < 1e-300) { Fc = 1e-300; dFc = 0.0; }")
+                out.append("    const double logFc = log10(Fc);")
+                out.append("    const double cc = -0.4 - 0.67 * logFc, nn = 0.75 - 1.27 * logFc;")
+                out.append("    const double PrS = fmax(Pr, 1e-300);")
+                out.append("    const double u = log10(PrS) + cc, dd = nn - 0.14 * u, f1 = u / dd, den = 1.0 + f1 * f1;")
+                out.append("    const double F = exp(2.302585092994045684 * logFc / den);")
+                out.append("    const double df1du = nn / (dd * dd), df1dl = (-0.67 * dd + 1.1762 * u) / (dd * dd);")
+                out.append("    const double dlFdlPr = -logFc * 2.0 * f1 / (den * den) * df1du;")
+                out.append("    const double dlFdlFc = 1.0 / den - logFc * 2.0 * f1 / (den * den) * df1dl;")
+                out.append("    const double dFdPr = F * dlFdlPr / PrS, dFdT = F * dlFdlFc * dFc / Fc;")
+            else:
+                out.append("    const double F = 1.0, dFdPr = 0.0, dFdT = 0.0;")
+            out.append("    const double gg = Pr / (1.0 + Pr), dg = 1.0 / ((1.0 + Pr) * (1.0 + Pr));")
+            out.append("    const double keff = kf * gg * F;")
+            out.append("    const double dkeffdM = kf * (dg * F + gg * dFdPr) * dPrdM;")
+            out.append("    const double dkeffdT = keff * dlnkf + kf * (dg * F + gg * dFdPr) * dPrdT + kf * gg * dFdT;")
+        pf = _prod(_power_terms(r.reactants))
+        out.append(f"    const double pf = {pf};")
+        if r.reversible:
+            gs, hs, dnu = _kc_exprs(r)
+            pr = _prod(_power_terms(r.products))
+            out.append(f"    const double pr = {pr};")
+            out.append(f"    const double invKc = exp({gs}){_refc_factor(dnu)};")
+            out.append(f"    const double dlnKc = ({hs} - {_d(dnu)}) * invT;")
+            out.append("    const double net = pf - pr * invKc;")
+            out.append("    const double dqdT = dkeffdT * net + keff * pr * invKc * dlnKc;")
+        else:
+            out.append("    const double net = pf;")
+            out.append("    const double dqdT = dkeffdT * net;")
+        out.append("    const double q = keff * net;")
+        # species stoichiometry of this reaction
+        nu: dict = {}
+        for i, c in r.reactants.items():
+            nu[i] = nu.get(i, 0) - c
+        for i, c in r.products.items():
+            nu[i] = nu.get(i, 0) + c
+        nu = {i: v for i, v in nu.items() if v != 0}
+        for i, v in sorted(nu.items()):
+            out.append(f"    wd[{i}] += {_d(v)} * q; dwdT[{i}] += {_d(v)} * dqdT;")
+        # dq/dC_k from the mass-action products
+        dq: dict = {}
+        for k in sorted(r.reactants):
+            coef, terms = _dpower_terms(r.reactants, k)
+            e = f"{_d(coef)} * {_prod(terms)}" if coef != 1.0 else _prod(terms)
+            dq.setdefault(k, []).append(f"keff * {e}")
+        if r.reversible:
+            for k in sorted(r.products):
+                coef, terms = _dpower_terms(r.products, k)
+                e = f"{_d(coef)} * {_prod(terms)}" if coef != 1.0 else _prod(terms)
+                dq.setdefault(k, []).append(f"- keff * invKc * {e}")
+        if r.has_third_body:
+            out.append("    const double tb = dkeffdM * net;")
+            for k in range(ns):
+                e = r.efficiencies.get(k, 1.0)
+                if e != 0.0:
+                    dq.setdefault(k, []).append(f"{_d(e)} * tb" if e != 1.0 else "tb")
+        for k in sorted(dq):
+            expr = " + ".join(dq[k]).replace("+ -", "-")
+            out.append(f"    {{ const double d = {expr};")
+            for i, v in sorted(nu.items()):
+                out.append(f"      acc({i}, {k}, {_d(v * W[i] / W[k])} * d);")
+            out.append("    }")
+        out.append("  }")
+    out.append("}")
+    return "\n".join(out)
+
+
+def write_inc(mech: M.Mechanism, path: str) -> None:
+    ns = mech.nspec
+    name = mech.name
+    cn = _cname(name)
+    lines = [f"// GENERATED by pelelm_amd/mechanisms/hipgen.py from {name}.yaml -- do not edit.",
+             f"// {ns} species, {mech.nreac} reactions.", "#pragma once", "",
+             f"struct mech_{cn} {{", "",
+             f"static constexpr int NS = {ns};", f"static constexpr int NR = {mech.nreac};",
+             f"static constexpr int N1 = {ns + 1};",
+             f"static constexpr const char* NAME = \"{name}\";",
+             f"static constexpr double RU = {_d(M.R_CGS)};", f"static constexpr double PATM = {_d(M.P_ATM)};",
+             "static __host__ __device__ __forceinline__ double mw(int k) {",
+             "  constexpr double t[NS] = {" + ", ".join(_d(w) for w in mech.W) + "};",
+             "  return t[k];", "}",
+             "static __host__ __device__ __forceinline__ double imw(int k) {",
+             "  constexpr double t[NS] = {" + ", ".join(_d(1.0 / w) for w in mech.W) + "};",
+             "  return t[k];", "}",
+             "static const char* species(int k) {",
+             "  static const char* t[NS] = {" + ", ".join(f'"{s.name}"' for s in mech.species) + "};",
+             "  return t[k];", "}",
+             ""]
+    lines.append(gen_thermo(mech))
+    lines.append("")
+    lines.append(gen_wdot(mech))
+    lines.append("")
+    lines.append(gen_jacobian(mech))
+    lines.append("")
+    lines.append(f"}};  // struct mech_{cn}")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

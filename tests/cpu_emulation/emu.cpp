@@ -1,0 +1,49 @@
+// Runs the reactor kernels' device code serially on the CPU (see hip/hip_runtime.h
+// in this directory) so tests can check the kernel logic against the oracle without a GPU.
+#define PELE_WAVES_PER_EU(n)
+#include "../../pelelm_amd/csrc/reactor_bdf.hip"
+
+thread_local Idx3 threadIdx, blockIdx, blockDim;
+
+using namespace pelelm;
+
+extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, double* rE, double* FC, double dt,
+                         double rtol, double atol, int max_order, int compact, unsigned long long* counters,
+                         long* trips) {
+  const int m = mech_lookup(mech);
+  if (m < 0) return -1;
+  const int N = mech_nspec(m);
+  ReactArgs a{};
+  a.rY.p = rY; a.rY.ns = ncell;
+  a.T.p = T; a.T.ns = ncell;
+  a.rE.p = rE; a.rE.ns = ncell;
+  a.FC.p = FC; a.FC.ns = ncell;
+  a.nx = (int)ncell; a.ny = 1; a.ncell = ncell;
+  a.dt = dt; a.rtol = rtol;
+  for (int i = 0; i <= N; ++i) a.atol[i] = atol;
+  a.qmax = max_order; a.mxstep = 20000;
+  const long nw = (ncell + 63) / 64;
+  double* ws = new double[(size_t)nw * ws_slots(m) * 64];
+  int* iws = new int[(size_t)nw * iws_slots(m) * 64];
+  int* lists = new int[4 * (size_t)nw * 64];
+  unsigned int counts[8] = {0};
+  a.ws = ws; a.iws = iws;
+  for (int i = 0; i < 2; ++i) {
+    a.list[i] = lists + (size_t)i * nw * 64;
+    a.slist[i] = lists + (size_t)(2 + i) * nw * 64;
+  }
+  a.counts = counts;
+  a.compact = compact;
+  a.counters = counters;
+  const hipError_t e = run_react(m, a, nullptr, 3 * 20000 + 100, trips);
+  delete[] ws;
+  delete[] iws;
+  delete[] lists;
+  return (int)e;
+}
+
+extern "C" int emu_rates(const char* mech, long ncell, const double* rY, const double* rH, double* T, double* out) {
+  const int m = mech_lookup(mech);
+  if (m < 0) return -1;
+  return (int)launch_rates(m, rY, rH, T, nullptr, out, ncell, nullptr);
+}

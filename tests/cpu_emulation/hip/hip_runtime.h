@@ -1,0 +1,58 @@
+// Host stand-in for <hip/hip_runtime.h> used only to run the reactor's device code
+// serially on the CPU (one lane at a time) in tests.  Cross-lane operations assume a
+// single active lane, so only the one-thread-per-cell schedule is meaningful here.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#define __device__
+#define __host__
+#define __global__
+#define __forceinline__ inline
+#define __noinline__ __attribute__((noinline))
+#define __launch_bounds__(x)
+
+using std::isfinite;
+
+typedef int hipError_t;
+typedef void* hipStream_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+  std::memcpy(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+
+struct dim3 {
+  unsigned x, y, z;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct Idx3 { unsigned x = 0, y = 0, z = 0; };
+extern thread_local Idx3 threadIdx, blockIdx, blockDim;
+
+template <class T> inline T max(T a, T b) { return a > b ? a : b; }
+template <class T> inline T min(T a, T b) { return a < b ? a : b; }
+inline int __lane_id() { return 0; }   // every emulated lane reduces on its own
+template <class T> inline T __shfl_down(T v, int, int = 64) { return T(0); }
+template <class T> inline T __shfl(T v, int, int = 64) { return v; }
+inline unsigned long long __ballot(int p) { return p ? (1ull << __lane_id()) : 0ull; }
+inline int __ffsll(long long x) { return __builtin_ffsll(x); }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+template <class T> inline T atomicAdd(T* p, T v) { T o = *p; *p += v; return o; }
+template <class T> inline T atomicMax(T* p, T v) { T o = *p; if (v > o) *p = v; return o; }
+
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)              \
+  do {                                                                          \
+    const dim3 g_ = (grid), b_ = (block);                                       \
+    blockDim.x = b_.x;                                                          \
+    for (unsigned bx_ = 0; bx_ < g_.x; ++bx_)                                   \
+      for (unsigned tx_ = 0; tx_ < b_.x; ++tx_) {                               \
+        blockIdx.x = bx_;                                                       \
+        threadIdx.x = tx_;                                                      \
+        kernel(__VA_ARGS__);                                                    \
+      }                                                                         \
+  } while (0)

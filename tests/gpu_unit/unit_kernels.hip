@@ -1,0 +1,310 @@
+// Bounded unit kernels for the reactor's device components (right-hand side,
+// analytic Jacobian, LU factor/solve, BDF coefficients, Nordsieck order changes).
+// The same file compiles for the GPU with hipcc and for the CPU with the serial
+// stand-in under tests/cpu_emulation, so tests can compare the two bit for bit.
+// Every kernel runs a fixed amount of work per lane.
+#include <vector>
+
+#include "../../pelelm_amd/csrc/reactor_bdf.hip"
+
+#ifndef UNIT_HOST
+#define UNIT_DEV_ALLOC(p, n) (void)hipMalloc((void**)&(p), (n))
+#define UNIT_DEV_FREE(p) (void)hipFree(p)
+#define UNIT_H2D(d, h, n) (void)hipMemcpy((d), (h), (n), hipMemcpyHostToDevice)
+#define UNIT_D2H(h, d, n) (void)hipMemcpy((h), (d), (n), hipMemcpyDeviceToHost)
+#define UNIT_SYNC() (void)hipDeviceSynchronize()
+#else
+#include <cstdlib>
+thread_local Idx3 threadIdx, blockIdx, blockDim;
+#define UNIT_DEV_ALLOC(p, n) (p) = (decltype(p))std::malloc(n)
+#define UNIT_DEV_FREE(p) std::free(p)
+#define UNIT_H2D(d, h, n) std::memcpy((d), (h), (n))
+#define UNIT_D2H(h, d, n) std::memcpy((h), (d), (n))
+#define UNIT_SYNC()
+#endif
+
+namespace pelelm {
+namespace {
+
+// in:  y [ncell][N1], t, rE0, fH per cell; out: f [ncell][N1], Tsol
+template <class M>
+__global__ void k_rhs(const double* y, const double* t, const double* rE0, const double* fH, const double* fy,
+                      double* f, double* Tsol, double* ws, long ncell) {
+  constexpr int N1 = M::N1;
+  using S = Slots<N1>;
+  const long c = (long)blockIdx.x * 64 + threadIdx.x;
+  if (c >= ncell) return;
+  WS w{ws + (c >> 6) * (long)S::TOTAL * 64 + (c & 63)};
+  for (int k = 0; k < N1 - 1; ++k) w[S::FY + k] = fy[c * (N1 - 1) + k];
+  double yy[N1], ff[N1], T;
+  for (int i = 0; i < N1; ++i) yy[i] = y[c * N1 + i];
+  rhs<M>(yy, t[c], w, rE0[c], fH[c], ff, T);
+  for (int i = 0; i < N1; ++i) f[c * N1 + i] = ff[i];
+  Tsol[c] = T;
+}
+
+// Lanes with (lane % 3 != 0) evaluate the Jacobian at (y, T, fT), factor I - gamma J and
+// solve for b; every lane carries a live vector through the branch and writes it back.
+template <class M>
+__global__ void k_jac_lu(const double* y, const double* T, const double* fT, const double* gamma,
+                         const double* b, double* J, double* x, double* keep_out, int* sing, double* ws, int* iws,
+                         long ncell) {
+  constexpr int N1 = M::N1;
+  using S = Slots<N1>;
+  const long c = (long)blockIdx.x * 64 + threadIdx.x;
+  if (c >= ncell) return;
+  WS w{ws + (c >> 6) * (long)S::TOTAL * 64 + (c & 63)};
+  IWS iw{iws + (c >> 6) * (long)S::ITOTAL * 64 + (c & 63)};
+  double keep[N1];
+  for (int i = 0; i < N1; ++i) keep[i] = y[c * N1 + i] * 3.0 + (double)c;
+  // a private array indexed with a run-time value (so it lives in scratch) and a full
+  // lane state, both live across the outlined calls below
+  double dyn[8];
+  const int idx = (int)(c * 7 % 8);
+  for (int i = 0; i < 8; ++i) dyn[i] = (double)(c * 8 + i);
+  dyn[idx] += 0.5;
+  Lane<M> st;
+  st.ws.p = ws;
+  for (int i = 0; i < 6; ++i) st.tq[i] = (double)(c + i);
+  for (int i = 0; i < LMAX; ++i) st.l[i] = (double)(2 * c + i);
+  for (int i = 0; i <= LMAX; ++i) st.tau[i] = (double)(3 * c + i);
+  st.q = 1 + (int)(c % 4);
+  int sg = -1;
+  if ((threadIdx.x % 3) != 0) {
+    double yy[N1];
+    for (int i = 0; i < N1; ++i) yy[i] = y[c * N1 + i];
+    jacobian<M>(yy, T[c], fT[c], w);
+    sg = lu_factor<N1>(w, iw, gamma[c]);
+    for (int i = 0; i < N1; ++i) w[S::TMP + i] = b[c * N1 + i];
+    double xx[N1];
+    lu_solve<N1>(w, iw, xx);
+    for (int i = 0; i < N1; ++i) x[c * N1 + i] = xx[i];
+    for (int e = 0; e < N1 * N1; ++e) J[c * N1 * N1 + e] = w[S::JS + e];
+  }
+  sing[c] = sg;
+  for (int i = 0; i < N1; ++i) keep_out[c * N1 + i] = keep[i];
+  // private state after the calls: dyn[0..7], tq[q], l[q], tau[q+1]
+  double chk = 0.0;
+  for (int i = 0; i < 8; ++i) chk += dyn[(idx + i) & 7] * (double)(i + 1);
+  keep_out[ncell * N1 + c * 4 + 0] = chk;
+  keep_out[ncell * N1 + c * 4 + 1] = st.tq[st.q];
+  keep_out[ncell * N1 + c * 4 + 2] = st.l[st.q];
+  keep_out[ncell * N1 + c * 4 + 3] = st.tau[st.q + 1];
+}
+
+// BDF coefficient / Nordsieck logic of one lane: set_coeffs for given (q, h, tau,
+// qwait, nst, gammap), then an order increase and decrease on a given history.
+// scal in : [q, qwait, nst, qmax, h, hscale, gammap, tau1..tau6]   (13 doubles)
+// out     : l[0..5], tq[1..5], gamma, gamrat, rl1, zn after increase (6*N1), after decrease
+template <class M>
+__global__ void k_coeffs(const double* scal, const double* zn_in, double* out, double* ws, long ncell) {
+  constexpr int N1 = M::N1;
+  using S = Slots<N1>;
+  const long c = (long)blockIdx.x * 64 + threadIdx.x;
+  if (c >= ncell) return;
+  Lane<M> st;
+  st.ws.p = ws + (c >> 6) * (long)S::TOTAL * 64 + (c & 63);
+  const double* s = scal + c * 13;
+  st.q = (int)s[0];
+  st.qwait = (int)s[1];
+  st.nst = (long)s[2];
+  st.qmax = (int)s[3];
+  st.h = s[4];
+  st.hscale = s[5];
+  st.gammap = s[6];
+  st.tau[0] = 0.0;
+  for (int i = 1; i <= LMAX; ++i) st.tau[i] = s[6 + i];
+  for (int i = 0; i < 6; ++i) st.tq[i] = -1.0;       // tq[1], tq[3] are only set when qwait == 1
+  for (int i = 0; i < LMAX; ++i) st.l[i] = -1.0;
+  st.L = st.q + 1;
+  st.set_coeffs();
+  double* o = out + c * (6 + 5 + 3 + 12 * N1);
+  for (int i = 0; i < 6; ++i) o[i] = (i <= st.q) ? st.l[i] : 0.0;
+  for (int i = 1; i <= 5; ++i) o[5 + i] = st.tq[i];
+  o[11] = st.gamma;
+  o[12] = st.gamrat;
+  o[13] = st.rl1;
+  for (int j = 0; j < LMAX; ++j)
+    for (int i = 0; i < N1; ++i) st.zn(j, i) = zn_in[(c * LMAX + j) * N1 + i];
+  if (st.q < st.qmax) {
+    st.L = st.q + 1;
+    st.increase_bdf();
+  }
+  for (int j = 0; j < LMAX; ++j)
+    for (int i = 0; i < N1; ++i) o[14 + j * N1 + i] = st.zn(j, i);
+  for (int j = 0; j < LMAX; ++j)
+    for (int i = 0; i < N1; ++i) st.zn(j, i) = zn_in[(c * LMAX + j) * N1 + i];
+  if (st.q > 1) st.decrease_bdf();
+  for (int j = 0; j < LMAX; ++j)
+    for (int i = 0; i < N1; ++i) o[14 + (LMAX + j) * N1 + i] = st.zn(j, i);
+}
+
+template <class T>
+T* dev_copy(const T* h, size_t n) {
+  T* d = nullptr;
+  UNIT_DEV_ALLOC(d, n * sizeof(T));
+  UNIT_H2D(d, h, n * sizeof(T));
+  return d;
+}
+template <class T>
+T* dev_zero(size_t n) {
+  T* d = nullptr;
+  UNIT_DEV_ALLOC(d, n * sizeof(T));
+  std::vector<T> z(n, T(0));
+  UNIT_H2D(d, z.data(), n * sizeof(T));
+  return d;
+}
+
+template <class M>
+int unit_rhs_t(long n, const double* y, const double* t, const double* rE0, const double* fH, const double* fy,
+               double* f, double* T) {
+  constexpr int N1 = M::N1;
+  const long nw = (n + 63) / 64;
+  double *dy = dev_copy(y, n * N1), *dt = dev_copy(t, n), *de = dev_copy(rE0, n), *dh = dev_copy(fH, n);
+  double *dfy = dev_copy(fy, n * (N1 - 1)), *df = dev_zero<double>(n * N1), *dT = dev_zero<double>(n);
+  double* ws = dev_zero<double>(nw * Slots<N1>::TOTAL * 64);
+  hipLaunchKernelGGL(k_rhs<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, dy, dt, de, dh, dfy, df, dT, ws, n);
+  UNIT_SYNC();
+  UNIT_D2H(f, df, n * N1 * sizeof(double));
+  UNIT_D2H(T, dT, n * sizeof(double));
+  for (void* p : {(void*)dy, (void*)dt, (void*)de, (void*)dh, (void*)dfy, (void*)df, (void*)dT, (void*)ws})
+    UNIT_DEV_FREE(p);
+  return 0;
+}
+
+template <class M>
+int unit_jac_lu_t(long n, const double* y, const double* T, const double* fT, const double* gamma, const double* b,
+                  double* J, double* x, double* keep, int* sing) {
+  constexpr int N1 = M::N1;
+  const long nw = (n + 63) / 64;
+  double *dy = dev_copy(y, n * N1), *dT = dev_copy(T, n), *dfT = dev_copy(fT, n), *dg = dev_copy(gamma, n);
+  double *db = dev_copy(b, n * N1), *dJ = dev_zero<double>(n * N1 * N1), *dx = dev_zero<double>(n * N1);
+  double* dk = dev_zero<double>(n * N1 + 4 * n);
+  int* ds = dev_zero<int>(n);
+  double* ws = dev_zero<double>(nw * Slots<N1>::TOTAL * 64);
+  int* iws = dev_zero<int>(nw * Slots<N1>::ITOTAL * 64);
+  hipLaunchKernelGGL(k_jac_lu<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, dy, dT, dfT, dg, db, dJ, dx, dk, ds, ws,
+                     iws, n);
+  UNIT_SYNC();
+  UNIT_D2H(J, dJ, n * N1 * N1 * sizeof(double));
+  UNIT_D2H(x, dx, n * N1 * sizeof(double));
+  UNIT_D2H(keep, dk, (n * N1 + 4 * n) * sizeof(double));
+  UNIT_D2H(sing, ds, n * sizeof(int));
+  for (void* p : {(void*)dy, (void*)dT, (void*)dfT, (void*)dg, (void*)db, (void*)dJ, (void*)dx, (void*)dk, (void*)ds,
+                  (void*)ws, (void*)iws})
+    UNIT_DEV_FREE(p);
+  return 0;
+}
+
+template <class M>
+int unit_coeffs_t(long n, const double* scal, const double* zn, double* out) {
+  constexpr int N1 = M::N1;
+  const long nw = (n + 63) / 64;
+  const long no = n * (14 + 12 * N1);
+  double *ds = dev_copy(scal, n * 13), *dz = dev_copy(zn, n * LMAX * N1), *dout = dev_zero<double>(no);
+  double* ws = dev_zero<double>(nw * Slots<N1>::TOTAL * 64);
+  hipLaunchKernelGGL(k_coeffs<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, ds, dz, dout, ws, n);
+  UNIT_SYNC();
+  UNIT_D2H(out, dout, no * sizeof(double));
+  for (void* p : {(void*)ds, (void*)dz, (void*)dout, (void*)ws}) UNIT_DEV_FREE(p);
+  return 0;
+}
+
+
+// Step-size / order control of one lane after a successful step (prepare_next_step,
+// including pow-based eta estimates) and after a failed error test's first retry.
+// scal in : [q, qmax, qwait, etamax, h, dsm, tq1..tq5, saved_tq5, tau1..tau6]  (18)
+// vectors : zn (LMAX*N1), acor (N1), ewt (N1)
+// out     : [eta, qprime, hprime, qwait, zn[qmax] after (N1)], then pow(x, y) table
+template <class M>
+__global__ void k_stepctl(const double* scal, const double* zn_in, const double* acor, const double* ewt,
+                          double* out, const double* px, const double* py, double* pz, double* ws, long ncell) {
+  constexpr int N1 = M::N1;
+  using S = Slots<N1>;
+  const long c = (long)blockIdx.x * 64 + threadIdx.x;
+  if (c >= ncell) return;
+  Lane<M> st;
+  st.ws.p = ws + (c >> 6) * (long)S::TOTAL * 64 + (c & 63);
+  const double* s = scal + c * 18;
+  st.q = (int)s[0];
+  st.qmax = (int)s[1];
+  st.qwait = (int)s[2];
+  st.etamax = s[3];
+  st.h = s[4];
+  st.dsm = s[5];
+  st.tq[0] = 0.0;
+  for (int i = 1; i <= 5; ++i) st.tq[i] = s[5 + i];
+  st.saved_tq5 = s[11];
+  st.tau[0] = 0.0;
+  for (int i = 1; i <= LMAX; ++i) st.tau[i] = s[11 + i];
+  st.L = st.q + 1;
+  st.eta = -1.0;
+  st.qprime = -1;
+  st.hprime = -1.0;
+  for (int j = 0; j < LMAX; ++j)
+    for (int i = 0; i < N1; ++i) st.zn(j, i) = zn_in[(c * LMAX + j) * N1 + i];
+  for (int i = 0; i < N1; ++i) {
+    st.ws[S::ACOR + i] = acor[c * N1 + i];
+    st.ws[S::EWT + i] = ewt[c * N1 + i];
+  }
+  st.prepare_next_step();
+  double* o = out + c * (4 + N1);
+  o[0] = st.eta;
+  o[1] = (double)st.qprime;
+  o[2] = st.hprime;
+  o[3] = (double)st.qwait;
+  for (int i = 0; i < N1; ++i) o[4 + i] = st.zn(st.qmax, i);
+  pz[c] = pow(px[c], py[c]);
+}
+
+template <class M>
+int unit_stepctl_t(long n, const double* scal, const double* zn, const double* acor, const double* ewt, double* out,
+                   const double* px, const double* py, double* pz) {
+  constexpr int N1 = M::N1;
+  const long nw = (n + 63) / 64;
+  double *ds = dev_copy(scal, n * 18), *dz = dev_copy(zn, n * LMAX * N1), *da = dev_copy(acor, n * N1);
+  double *de = dev_copy(ewt, n * N1), *dout = dev_zero<double>(n * (4 + N1));
+  double *dpx = dev_copy(px, n), *dpy = dev_copy(py, n), *dpz = dev_zero<double>(n);
+  double* ws = dev_zero<double>(nw * Slots<N1>::TOTAL * 64);
+  hipLaunchKernelGGL(k_stepctl<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, ds, dz, da, de, dout, dpx, dpy, dpz, ws,
+                     n);
+  UNIT_SYNC();
+  UNIT_D2H(out, dout, n * (4 + N1) * sizeof(double));
+  UNIT_D2H(pz, dpz, n * sizeof(double));
+  for (void* p : {(void*)ds, (void*)dz, (void*)da, (void*)de, (void*)dout, (void*)dpx, (void*)dpy, (void*)dpz,
+                  (void*)ws})
+    UNIT_DEV_FREE(p);
+  return 0;
+}
+}  // namespace
+}  // namespace pelelm
+
+using namespace pelelm;
+
+extern "C" int unit_rhs(const char* mech, long n, const double* y, const double* t, const double* rE0,
+                        const double* fH, const double* fy, double* f, double* T) {
+  const int m = mech_lookup(mech);
+  if (m == MECH_DRM19) return unit_rhs_t<mech_drm19>(n, y, t, rE0, fH, fy, f, T);
+  if (m == MECH_LIDRYER) return unit_rhs_t<mech_LiDryer>(n, y, t, rE0, fH, fy, f, T);
+  return -1;
+}
+extern "C" int unit_jac_lu(const char* mech, long n, const double* y, const double* T, const double* fT,
+                           const double* gamma, const double* b, double* J, double* x, double* keep, int* sing) {
+  const int m = mech_lookup(mech);
+  if (m == MECH_DRM19) return unit_jac_lu_t<mech_drm19>(n, y, T, fT, gamma, b, J, x, keep, sing);
+  if (m == MECH_LIDRYER) return unit_jac_lu_t<mech_LiDryer>(n, y, T, fT, gamma, b, J, x, keep, sing);
+  return -1;
+}
+extern "C" int unit_coeffs(const char* mech, long n, const double* scal, const double* zn, double* out) {
+  const int m = mech_lookup(mech);
+  if (m == MECH_DRM19) return unit_coeffs_t<mech_drm19>(n, scal, zn, out);
+  if (m == MECH_LIDRYER) return unit_coeffs_t<mech_LiDryer>(n, scal, zn, out);
+  return -1;
+}
+extern "C" int unit_stepctl(const char* mech, long n, const double* scal, const double* zn, const double* acor,
+                            const double* ewt, double* out, const double* px, const double* py, double* pz) {
+  const int m = mech_lookup(mech);
+  if (m == MECH_DRM19) return unit_stepctl_t<mech_drm19>(n, scal, zn, acor, ewt, out, px, py, pz);
+  if (m == MECH_LIDRYER) return unit_stepctl_t<mech_LiDryer>(n, scal, zn, acor, ewt, out, px, py, pz);
+  return -1;
+}

@@ -1,0 +1,195 @@
+"""Device components of the reactor on the GPU against the same code on the CPU.
+
+tests/gpu_unit/unit_kernels.hip wraps the reactor's device functions (right-hand side
+with HY2T, analytic Jacobian, LU factor/solve of I - gamma J, BDF coefficients and
+Nordsieck order changes) in bounded kernels; libunit_gpu.so is that file built for
+gfx950, libunit_cpu.so the same file built for the host through the serial stand-in.
+The CPU twin is itself checked against the oracle in test_kernel_emulation.py.
+Jacobian/LU calls run in divergent lanes (lane % 3 != 0) while every lane carries a
+live vector across the branch, which must come back intact.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pelelm_amd import cells
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+UNIT = os.path.join(HERE, "gpu_unit")
+
+
+def _load(name):
+    path = os.path.join(UNIT, name)
+    if not os.path.exists(path):
+        pytest.skip(f"{path} not built")
+    return C.CDLL(path)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _inputs(mech, wl, n, seed=3):
+    d = cells.generate_workload(wl, ncell=n)
+    m = O.Mech(mech)
+    N = m.nspec
+    N1 = N + 1
+    rng = np.random.default_rng(seed)
+    y = np.ascontiguousarray(np.concatenate([d["rY"].T, d["T"][:, None] + rng.normal(0, 5, n)[:, None]], axis=1))
+    t = rng.uniform(0, 1e-6, n)
+    rE0 = d["rE"].copy()
+    fH = rng.normal(0, 1e7, n)
+    fy = np.ascontiguousarray(rng.normal(0, 1e-4, (n, N)) * d["rY"].T)
+    return m, N1, y, t, rE0, fH, fy
+
+
+def run_rhs(L, mech, y, t, rE0, fH, fy):
+    n, N1 = y.shape
+    f = np.zeros((n, N1))
+    T = np.zeros(n)
+    assert L.unit_rhs(mech.encode(), C.c_long(n), _p(y), _p(t), _p(rE0), _p(fH), _p(fy), _p(f), _p(T)) == 0
+    return f, T
+
+
+def run_jac_lu(L, mech, y, T, fT, gamma, b):
+    n, N1 = y.shape
+    J = np.zeros((n, N1, N1))
+    x = np.zeros((n, N1))
+    keep = np.zeros(n * N1 + 4 * n)
+    sing = np.zeros(n, dtype=np.int32)
+    assert L.unit_jac_lu(mech.encode(), C.c_long(n), _p(y), _p(T), _p(fT), _p(gamma), _p(b), _p(J), _p(x),
+                         _p(keep), _p(sing)) == 0
+    return J, x, keep, sing
+
+
+def _dump(name, **arrays):
+    """Keep the compared arrays for inspection when a GPU run writes gpurun_out/."""
+    out = os.path.join(os.path.dirname(HERE), "gpurun_out")
+    if os.path.isdir(out):
+        np.savez(os.path.join(out, f"unit_{name}.npz"), **arrays)
+
+
+def coeff_inputs(n, N1, seed=5):
+    rng = np.random.default_rng(seed)
+    scal = np.zeros((n, 13))
+    scal[:, 0] = rng.integers(1, 5, n)          # q in 1..4
+    scal[:, 1] = rng.integers(0, 4, n)          # qwait
+    scal[:, 2] = rng.choice([0, 3, 25], n)      # nst
+    scal[:, 3] = 4                              # qmax
+    h = 10 ** rng.uniform(-12, -7, n)
+    scal[:, 4] = h
+    scal[:, 5] = h * rng.uniform(0.8, 1.2, n)
+    scal[:, 6] = h * rng.uniform(0.5, 2.0, n)
+    scal[:, 7:13] = h[:, None] * rng.uniform(0.3, 3.0, (n, 6))
+    zn = rng.normal(size=(n, 6, N1))
+    return scal, zn
+
+
+def run_coeffs(L, mech, scal, zn):
+    n = scal.shape[0]
+    N1 = zn.shape[2]
+    out = np.zeros((n, 14 + 12 * N1))
+    assert L.unit_coeffs(mech.encode(), C.c_long(n), _p(scal), _p(np.ascontiguousarray(zn)), _p(out)) == 0
+    return out
+
+
+CASES = [("LiDryer", "lidryer_16"), ("drm19", "drm19_64")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mech,wl", CASES)
+def test_rhs_gpu_matches_cpu(mech, wl):
+    G, Cl = _load("libunit_gpu.so"), _load("libunit_cpu.so")
+    m, N1, y, t, rE0, fH, fy = _inputs(mech, wl, 640)
+    fg, Tg = run_rhs(G, mech, y, t, rE0, fH, fy)
+    fc, Tc = run_rhs(Cl, mech, y, t, rE0, fH, fy)
+    np.testing.assert_allclose(Tg, Tc, rtol=1e-13)
+    scale = np.abs(fc).max(axis=1, keepdims=True)
+    assert np.max(np.abs(fg - fc) / scale) < 1e-11
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mech,wl", CASES)
+def test_jac_lu_gpu_matches_cpu_in_divergent_lanes(mech, wl):
+    G, Cl = _load("libunit_gpu.so"), _load("libunit_cpu.so")
+    m, N1, y, t, rE0, fH, fy = _inputs(mech, wl, 640)
+    n = y.shape[0]
+    _, T = run_rhs(Cl, mech, y, t, rE0, fH, fy)
+    f, _ = run_rhs(Cl, mech, y, t, rE0, fH, fy)
+    fT = f[:, -1].copy()
+    rng = np.random.default_rng(11)
+    gamma = 10 ** rng.uniform(-10, -6, n)
+    b = rng.normal(size=(n, N1))
+    Jg, xg, kg, sg = run_jac_lu(G, mech, y, T, fT, gamma, b)
+    Jc, xc, kc, sc = run_jac_lu(Cl, mech, y, T, fT, gamma, b)
+    _dump(f"jac_lu_{mech}", Jg=Jg, Jc=Jc, xg=xg, xc=xc, kg=kg, kc=kc, sg=sg, sc=sc)
+    active = (np.arange(n) % 64) % 3 != 0
+    # live state across the divergent calls: registers (1-ulp FMA contraction allowed) and
+    # private arrays indexed at run time, which must be untouched
+    np.testing.assert_allclose(kg[: n * N1], kc[: n * N1], rtol=1e-15)
+    np.testing.assert_array_equal(kg[n * N1:], kc[n * N1:])
+    np.testing.assert_array_equal(sg, sc)
+    assert np.all(sg[~active] == -1) and np.all(sg[active] == 0)
+    scale = np.abs(Jc[active]).max(axis=2, keepdims=True) + 1e-300
+    assert np.max(np.abs(Jg[active] - Jc[active]) / scale) < 1e-11
+    # near-singular systems (|x| up to 1e7) amplify rounding: compare against each solution's scale
+    xs = np.abs(xc[active]).max(axis=1, keepdims=True)
+    assert np.max(np.abs(xg[active] - xc[active]) / xs) < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mech", ["LiDryer", "drm19"])
+def test_bdf_coefficients_gpu_matches_cpu(mech):
+    G, Cl = _load("libunit_gpu.so"), _load("libunit_cpu.so")
+    N1 = {"LiDryer": 10, "drm19": 22}[mech]
+    scal, zn = coeff_inputs(256, N1)
+    og = run_coeffs(G, mech, scal, zn)
+    oc = run_coeffs(Cl, mech, scal, zn)
+    _dump(f"coeffs_{mech}", og=og, oc=oc, scal=scal, zn=zn)
+    np.testing.assert_allclose(og, oc, rtol=1e-12, atol=1e-300)
+
+
+def stepctl_inputs(n, N1, seed=9):
+    rng = np.random.default_rng(seed)
+    scal = np.zeros((n, 18))
+    scal[:, 0] = rng.integers(1, 5, n)                  # q
+    scal[:, 1] = 4                                      # qmax
+    scal[:, 2] = rng.integers(0, 3, n)                  # qwait
+    scal[:, 3] = rng.choice([1.0, 10.0, 1e4], n)        # etamax
+    h = 10 ** rng.uniform(-12, -7, n)
+    scal[:, 4] = h
+    scal[:, 5] = 10 ** rng.uniform(-6, 0, n)            # dsm
+    scal[:, 6:11] = rng.uniform(0.05, 2.0, (n, 5))      # tq1..5
+    scal[:, 11] = rng.choice([0.0, 1.0], n) * rng.uniform(0.1, 2, n)   # saved_tq5
+    scal[:, 12:18] = h[:, None] * rng.uniform(0.3, 3.0, (n, 6))
+    zn = rng.normal(size=(n, 6, N1)) * 1e-3
+    acor = rng.normal(size=(n, N1)) * 1e-4
+    ewt = 10 ** rng.uniform(2, 8, (n, N1))
+    px = 10 ** rng.uniform(-6, 1.5, n)
+    py = rng.choice([1 / 2, 1 / 3, 1 / 4, 1 / 5, 2.0, 3.0, 5.0], n)
+    return scal, zn, acor, ewt, px, py
+
+
+def run_stepctl(L, mech, scal, zn, acor, ewt, px, py):
+    n, N1 = acor.shape
+    out = np.zeros((n, 4 + N1))
+    pz = np.zeros(n)
+    assert L.unit_stepctl(mech.encode(), C.c_long(n), _p(scal), _p(np.ascontiguousarray(zn)), _p(acor), _p(ewt),
+                          _p(out), _p(px), _p(py), _p(pz)) == 0
+    return out, pz
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mech", ["LiDryer", "drm19"])
+def test_step_control_gpu_matches_cpu(mech):
+    G, Cl = _load("libunit_gpu.so"), _load("libunit_cpu.so")
+    N1 = {"LiDryer": 10, "drm19": 22}[mech]
+    args = stepctl_inputs(512, N1)
+    og, pg = run_stepctl(G, mech, *args)
+    oc, pc = run_stepctl(Cl, mech, *args)
+    _dump(f"stepctl_{mech}", og=og, oc=oc, pg=pg, pc=pc)
+    np.testing.assert_allclose(pg, pc, rtol=1e-15)
+    np.testing.assert_allclose(og, oc, rtol=1e-12, atol=1e-300)
