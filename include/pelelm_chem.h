@@ -60,6 +60,12 @@ typedef struct {
   long npred_fallback;  /* steps where the learned prediction was rejected      */
   double ms_kernel;     /* device time of the integration kernel(s), ms         */
   long ntrips;          /* right-hand-side trips the call took                  */
+  long nhy2t_iters;     /* Newton iterations of the T-from-enthalpy solves      */
+  /* with mi.profile = 1: device time (ms), launches and cells processed per kernel
+     kind [0] right-hand side, [1] Jacobian/LU set-up, [2] Newton/step logic, [3] other */
+  double kernel_ms[4];
+  long kernel_launches[4];
+  long kernel_cells[4];
 } mi_chem_stats;
 
 /* ReactorBase::create(name) -- Source/PeleLM.cpp:706.
@@ -78,7 +84,7 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
      cvode.solve_type (denseAJ_direct), peleLM.use_typ_vals_chem (0/1),
      mi.mechanism (drm19 | LiDryer), mi.schedule (queue: finished cells leave the work
      lists, default | plain: fixed cell-to-lane mapping), mi.lu (fp64), mi.ir_sweeps,
-     mi.on_fail (continue | abort). */
+     mi.on_fail (continue | abort), mi.profile (0 | 1: per-kernel event timing). */
 int mi_chem_set(void* reactor, const char* key, const char* value);
 
 /* ReactorBase::set_typ_vals_ode -- Source/PeleLM.cpp:1497-1516: n = nspec + 1 typical

@@ -36,6 +36,8 @@ struct Reactor {
   int lu_mode = LU_FP64;
   int ir_sweeps = 3;
   bool on_fail_continue = true;
+  bool profile = false;
+  KernelProf prof{};
   // device state
   int device = -1;
   int ncu = 0;
@@ -150,7 +152,8 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   } else {
     // every step takes at least one trip; 3 per step bounds retries generously
     const long max_trips = 3 * r->max_steps + 100;
-    HIPCHK(r, run_react(r->mech, a, stream, max_trips, &r->last_trips));
+    r->prof = KernelProf{};
+    HIPCHK(r, run_react(r->mech, a, stream, max_trips, &r->last_trips, r->profile ? &r->prof : nullptr));
   }
   HIPCHK(r, hipEventRecord(r->ev1, stream));
   r->last_stream = stream;
@@ -262,6 +265,9 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
   } else if (k == "mi.ir_sweeps") {
     if (!parse_long(value, &l) || l < 0 || l > 10) return fail(r, MI_EINVAL, "mi.ir_sweeps must be 0..10");
     r->ir_sweeps = (int)l;
+  } else if (k == "mi.profile") {
+    if (!parse_long(value, &l) || (l != 0 && l != 1)) return fail(r, MI_EINVAL, "mi.profile must be 0 or 1");
+    r->profile = (l == 1);
   } else if (k == "mi.on_fail") {
     if (v == "continue") r->on_fail_continue = true;
     else if (v == "abort") r->on_fail_continue = false;
@@ -363,10 +369,16 @@ int mi_chem_get_stats(void* reactor, mi_chem_stats* out) {
   out->max_steps_cell = (long)c[CNT_MAXSTEPS];
   out->nlu_refine = (long)c[CNT_IR];
   out->nlu_fallback = (long)c[CNT_LUFALLBACK];
+  out->nhy2t_iters = (long)c[CNT_HY2T];
   float ms = 0.f;
   HIPCHK(r, hipEventElapsedTime(&ms, r->ev0, r->ev1));
   out->ms_kernel = ms;
   out->ntrips = r->last_trips;
+  for (int kk = 0; kk < 4 && kk < NKK; ++kk) {
+    out->kernel_ms[kk] = r->prof.ms[kk];
+    out->kernel_launches[kk] = r->prof.launches[kk];
+    out->kernel_cells[kk] = r->prof.cells[kk];
+  }
   if (out->nfail > 0 && !r->on_fail_continue) return fail(r, MI_ECELLFAIL, "cells failed to integrate");
   return MI_OK;
 }

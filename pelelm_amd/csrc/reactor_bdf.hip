@@ -94,9 +94,10 @@ __device__ __forceinline__ long cell_off(const IntView& v, int i, int j, int k) 
 // right-hand side: T from (rho h)(t) by Newton, then rates and the energy equation
 // ---------------------------------------------------------------------------
 template <class M>
-__device__ __forceinline__ double hy2t(const double h, const double* Y, double T) {
+__device__ __forceinline__ double hy2t(const double h, const double* Y, double T, int* iters = nullptr) {
   double cpR[M::NS], hRT[M::NS];
-  for (int it = 0; it < HY2T_MAXIT; ++it) {
+  int it = 0;
+  for (; it < HY2T_MAXIT; ++it) {
     M::thermo_cph(T, cpR, hRT);
     double hT = 0.0, cp = 0.0;
 #pragma unroll
@@ -108,14 +109,15 @@ __device__ __forceinline__ double hy2t(const double h, const double* Y, double T
     cp *= M::RU;
     const double dT = (h - hT) / cp;
     T += dT;
-    if (fabs(dT) <= HY2T_RTOL * T) break;
+    if (fabs(dT) <= HY2T_RTOL * T) { ++it; break; }
   }
+  if (iters) *iters = it;
   return T;
 }
 
 template <class M>
 __device__ __forceinline__ void rhs(const double* __restrict__ y, const double t, const WS& ws, const double rE0,
-                                    const double fH, double* __restrict__ f, double& Tsol) {
+                                    const double fH, double* __restrict__ f, double& Tsol, int* hy2t_iters = nullptr) {
   constexpr int N = M::NS;
   using S = Slots<M::N1>;
   double rho = 0.0;
@@ -125,7 +127,7 @@ __device__ __forceinline__ void rhs(const double* __restrict__ y, const double t
   double Y[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) Y[k] = y[k] * irho;
-  const double T = hy2t<M>((rE0 + fH * t) * irho, Y, y[N]);
+  const double T = hy2t<M>((rE0 + fH * t) * irho, Y, y[N], hy2t_iters);
   Tsol = T;
   const double logT = log(T), invT = 1.0 / T;
   double C[N], g[N], wd[N];
@@ -889,22 +891,26 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) k_eval(const ReactArg
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
   const long idx = (long)blockIdx.x * 64 + threadIdx.x;
-  if (idx >= (long)*count) return;
-  const long c = list[idx];
-  const WS w = cell_ws<M>(a.ws, c);
-  Lane<M> st;
-  load_lane(st, w, cell_iws<M>(a.iws, c), a);
-  if (st.phase == PH_NEED_CELL) return;
-  double y[N1], f[N1], Tsol;
+  int iters = 0;
+  if (idx < (long)*count) {
+    const long c = list[idx];
+    const WS w = cell_ws<M>(a.ws, c);
+    Lane<M> st;
+    load_lane(st, w, cell_iws<M>(a.iws, c), a);
+    if (st.phase != PH_NEED_CELL) {
+      double y[N1], f[N1], Tsol;
 #pragma unroll
-  for (int i = 0; i < N1; ++i) y[i] = st.zn(0, i) + st.a1 * st.zn(1, i) + st.a2 * w[S::ACOR + i];
-  rhs<M>(y, st.teval, w, st.rE0, st.fH, f, Tsol);
+      for (int i = 0; i < N1; ++i) y[i] = st.zn(0, i) + st.a1 * st.zn(1, i) + st.a2 * w[S::ACOR + i];
+      rhs<M>(y, st.teval, w, st.rE0, st.fH, f, Tsol, &iters);
 #pragma unroll
-  for (int i = 0; i < N1; ++i) {
-    w[S::Y + i] = y[i];
-    w[S::F + i] = f[i];
+      for (int i = 0; i < N1; ++i) {
+        w[S::Y + i] = y[i];
+        w[S::F + i] = f[i];
+      }
+      w[S::TS] = Tsol;
+    }
   }
-  w[S::TS] = Tsol;
+  if (a.counters) wave_add(&a.counters[CNT_HY2T], (unsigned long long)iters);
 }
 
 // Jacobian (when stale) and LU of I - gamma J for the cells starting a Newton attempt
@@ -1142,9 +1148,57 @@ __global__ void null_kernel(const ReactArgs a, int nspec) {
 // counts[0..1]: active-cell lists of the current / next trip, counts[2..3]: their set-up
 // lists; without compaction counts[4] is the (fixed) list length and counts[5] the
 // number of cells still integrating.
+// Per-kernel timing: an event pair per launch, recorded asynchronously and read back
+// once the call has finished, so profiling adds no synchronisation inside the trips.
+struct Timer {
+  KernelProf* prof;
+  hipStream_t stream;
+  static constexpr int MAXEV = 4096;
+  hipEvent_t ev[2 * MAXEV];
+  int kind[MAXEV];
+  long cells[MAXEV];
+  int n = 0, created = 0;
+  hipError_t begin() {
+    if (!prof || n >= MAXEV) return hipSuccess;
+    while (created < 2 * (n + 1)) PELE_TRY(hipEventCreate(&ev[created++]));
+    return hipEventRecord(ev[2 * n], stream);
+  }
+  hipError_t end(int k, long c) {
+    if (!prof || n >= MAXEV) return hipSuccess;
+    PELE_TRY(hipEventRecord(ev[2 * n + 1], stream));
+    kind[n] = k;
+    cells[n] = c;
+    ++n;
+    return hipSuccess;
+  }
+  hipError_t collect() {
+    if (!prof || n == 0) return hipSuccess;
+    PELE_TRY(hipEventSynchronize(ev[2 * n - 1]));
+    for (int i = 0; i < n; ++i) {
+      float ms = 0.f;
+      PELE_TRY(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+      prof->ms[kind[i]] += ms;
+      prof->launches[kind[i]] += 1;
+      prof->cells[kind[i]] += cells[i];
+    }
+    return hipSuccess;
+  }
+  ~Timer() {
+    for (int i = 0; i < created; ++i) (void)hipEventDestroy(ev[i]);
+  }
+};
+
 template <class M>
-hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* trips_out) {
+hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* trips_out, KernelProf* prof) {
   const long n = a.ncell;
+  Timer* tmp = new Timer();
+  tmp->prof = prof;
+  tmp->stream = stream;
+  Timer& tm = *tmp;
+  struct Del {
+    Timer* t;
+    ~Del() { delete t; }
+  } del_{tmp};
   const unsigned nb = (unsigned)((n + 63) / 64);
   hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, stream, a.counts);
   PELE_TRY(hipGetLastError());
@@ -1170,28 +1224,40 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
       hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, nxt, 2 + nxt);
       PELE_TRY(hipGetLastError());
       const unsigned ga = (active + 63) / 64;
+      PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_eval<M>, dim3(ga), dim3(64), 0, stream, a, a.list[cur], &a.counts[cur]);
       PELE_TRY(hipGetLastError());
+      PELE_TRY(tm.end(KK_EVAL, active));
       if (nsetup > 0) {
+        PELE_TRY(tm.begin());
         hipLaunchKernelGGL(k_setup<M>, dim3((nsetup + 63) / 64), dim3(64), 0, stream, a, a.slist[cur],
                            &a.counts[2 + cur]);
         PELE_TRY(hipGetLastError());
+        PELE_TRY(tm.end(KK_SETUP, nsetup));
       }
+      PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_consume<M>, dim3(ga), dim3(64), 0, stream, a, a.list[cur], &a.counts[cur], a.list[nxt],
                          &a.counts[nxt], a.slist[nxt], &a.counts[2 + nxt]);
       PELE_TRY(hipGetLastError());
+      PELE_TRY(tm.end(KK_CONSUME, active));
       cur = nxt;
     } else {
       if (h_counts[5] == 0) break;
       hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, 5, 6);
       PELE_TRY(hipGetLastError());
+      PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_eval<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4]);
       PELE_TRY(hipGetLastError());
+      PELE_TRY(tm.end(KK_EVAL, h_counts[5]));
+      PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_setup<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4]);
       PELE_TRY(hipGetLastError());
+      PELE_TRY(tm.end(KK_SETUP, 0));
+      PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_consume<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4], a.list[0],
                          &a.counts[5], a.slist[0], &a.counts[6]);
       PELE_TRY(hipGetLastError());
+      PELE_TRY(tm.end(KK_CONSUME, h_counts[5]));
     }
   }
   if (trip == max_trips) {
@@ -1202,6 +1268,7 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
     PELE_TRY(hipGetLastError());
   }
   if (trips_out) *trips_out = trip;
+  PELE_TRY(tm.collect());
   return hipSuccess;
 }
 
@@ -1253,10 +1320,11 @@ long iws_slots(int mech) {
   }
 }
 
-hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out) {
+hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out,
+                     KernelProf* prof) {
   switch (mech) {
-    case MECH_LIDRYER: return run_react_t<mech_LiDryer>(a, stream, max_trips, trips_out);
-    case MECH_DRM19: return run_react_t<mech_drm19>(a, stream, max_trips, trips_out);
+    case MECH_LIDRYER: return run_react_t<mech_LiDryer>(a, stream, max_trips, trips_out, prof);
+    case MECH_DRM19: return run_react_t<mech_drm19>(a, stream, max_trips, trips_out, prof);
     default: return hipErrorInvalidValue;
   }
 }

@@ -55,7 +55,7 @@ struct ReactArgs {
 
 enum CounterIndex : int {
   CNT_CELLS = 0, CNT_NST, CNT_NFE, CNT_NJE, CNT_NSETUPS, CNT_NNI, CNT_NCFN, CNT_NETF, CNT_NFAIL,
-  CNT_MAXSTEPS, CNT_IR, CNT_LUFALLBACK, NCOUNTERS = 16
+  CNT_MAXSTEPS, CNT_IR, CNT_LUFALLBACK, CNT_HY2T, NCOUNTERS = 16
 };
 
 // Mechanism ids (order of the dispatch table in reactor_bdf.hip).
@@ -70,7 +70,17 @@ long iws_slots(int mech);       // ints per lane
 // Integrate every cell described by `a`: trips of bounded kernels on `stream`, driven
 // from the host until no cell is active or max_trips trips have run (cells still
 // active then are reported failed).  Synchronises the stream once per trip.
-hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out);
+// Optional per-kernel timing: when `prof` is non-null every launch of the trip
+// kernels is bracketed by events and the per-kind device time (ms), launch count and
+// cells processed are accumulated into prof[kind].
+enum KernelKind : int { KK_EVAL = 0, KK_SETUP, KK_CONSUME, KK_OTHER, NKK };
+struct KernelProf {
+  double ms[NKK];
+  long launches[NKK];
+  long cells[NKK];
+};
+hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out,
+                     KernelProf* prof);
 hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t stream);
 hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T, const int* mask, double* rYdot,
                         long ncell, hipStream_t stream);

@@ -18,7 +18,10 @@ add (eff - 1) C_k only for species with non-unit efficiency.
 """
 from __future__ import annotations
 
+import json
 import math
+import os
+import re
 
 from . import mechanism as M
 
@@ -331,6 +334,40 @@ def gen_jacobian(mech: M.Mechanism) -> str:
     return "\n".join(out)
 
 
+_TRANS = ("exp", "log10", "log", "pow", "sqrt")
+
+
+def count_ops(code: str) -> dict:
+    """Static operation count of generated straight-line code: every binary + - * /
+    as written counts one flop (so a fused multiply-add counts two), fmax/fmin one, and
+    exp / log / log10 / pow / sqrt are counted separately as transcendentals.  Loops in
+    the generated code are the fixed per-species ones ("for (int k = 0; k < NS; ...)"),
+    whose single-statement bodies are weighted by NS."""
+    flops = 0
+    trans = {t: 0 for t in _TRANS}
+    ns = None
+    for line in code.splitlines():
+        line = line.split("//")[0]
+        m = re.search(r"for \(int k = 0; k < (\d+); \+\+k\)(.*)", line)
+        weight = 1
+        if m:
+            weight = int(m.group(1))
+            line = m.group(2)
+        # drop the ternary coefficient selects of the NASA polynomials
+        line = re.sub(r"\(lo \? [^:]+ : [^)]+\)", "a", line)
+        for t in _TRANS:
+            n = len(re.findall(r"\b%s\(" % t, line))
+            trans[t] += n * weight
+        n = len(re.findall(r"\bfm(ax|in)\(", line))
+        # binary operators: + * / anywhere; - when it follows an operand
+        ops = len(re.findall(r"(?<=[\w\)\]\s])\s*[\+\*/](?!=)", line))
+        ops += len(re.findall(r"(?<=[\w\)\]])\s+-\s+", line))
+        ops += len(re.findall(r"[\+\-\*/]=", line))
+        flops += (ops + n) * weight
+    del ns
+    return {"flops": flops, "transcendentals": sum(trans.values()), "by_function": trans}
+
+
 def write_inc(mech: M.Mechanism, path: str) -> None:
     ns = mech.nspec
     name = mech.name
@@ -352,12 +389,30 @@ def write_inc(mech: M.Mechanism, path: str) -> None:
              "  static const char* t[NS] = {" + ", ".join(f'"{s.name}"' for s in mech.species) + "};",
              "  return t[k];", "}",
              ""]
-    lines.append(gen_thermo(mech))
+    th = gen_thermo(mech)
+    wd = gen_wdot(mech)
+    jac = gen_jacobian(mech)
+    parts = th.split("static __device__ __forceinline__ void ")
+    counts = {}
+    for part in parts[1:]:
+        fname = part.split("(")[0]
+        counts[fname] = count_ops(part)
+    counts["wdot"] = count_ops(wd)
+    counts["jacobian"] = count_ops(jac)
+    lines.append("// static operation counts (flops; transcendentals) of the generated functions")
+    for k, v in counts.items():
+        lines.append(f"static constexpr long OPS_{k.upper()} = {v['flops']};")
+        lines.append(f"static constexpr long TRANS_{k.upper()} = {v['transcendentals']};")
     lines.append("")
-    lines.append(gen_wdot(mech))
+    lines.append(th)
     lines.append("")
-    lines.append(gen_jacobian(mech))
+    lines.append(wd)
+    lines.append("")
+    lines.append(jac)
     lines.append("")
     lines.append(f"}};  // struct mech_{cn}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+    ops_path = os.path.join(os.path.dirname(os.path.dirname(path)), "..", "mechanisms", f"{name}_ops.json")
+    with open(os.path.normpath(ops_path), "w") as f:
+        json.dump({"mechanism": name, "nspec": ns, "nreac": mech.nreac, "functions": counts}, f, indent=1)

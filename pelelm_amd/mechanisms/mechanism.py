@@ -23,7 +23,7 @@ import yaml
 ATOMIC_WEIGHTS = {"H": 1.00797, "O": 15.9994, "C": 12.01115, "N": 14.0067, "AR": 39.948, "HE": 4.0026}
 
 # CHEMKIN-II gas constant.  The PMF profiles PeleLM ships satisfy the ideal-gas law
-# with this value to 1e-9 relative (Exec/RegTests/FlameSheet/drm19_pmf.dat rows), but
+# with this value to 3e-8 relative (Exec/RegTests/FlameSheet/drm19_pmf.dat rows), but
 # only to 5.7e-6 with the CODATA-2018 value, so the mechanism tables use it.
 R_CGS = 8.314510e7              # erg/(mol K)
 R_CAL = R_CGS / 4.184e7         # cal/(mol K)

@@ -38,7 +38,9 @@ class MiStats(C.Structure):
                 ("nsetups", C.c_long), ("nni", C.c_long), ("ncfn", C.c_long), ("netf", C.c_long),
                 ("nfail", C.c_long), ("max_steps_cell", C.c_long), ("nlu_refine", C.c_long),
                 ("nlu_fallback", C.c_long), ("npred_used", C.c_long), ("npred_fallback", C.c_long),
-                ("ms_kernel", C.c_double), ("ntrips", C.c_long)]
+                ("ms_kernel", C.c_double), ("ntrips", C.c_long), ("nhy2t_iters", C.c_long),
+                ("kernel_ms", C.c_double * 4),
+                ("kernel_launches", C.c_long * 4), ("kernel_cells", C.c_long * 4)]
 
 
 # Every exported entry point of include/pelelm_chem.h.
@@ -179,6 +181,8 @@ class ReactorBase:
         st = MiStats()
         rc = lib().mi_chem_get_stats(self._h, C.byref(st))
         out = {f: getattr(st, f) for f, _ in MiStats._fields_}
+        for f in ("kernel_ms", "kernel_launches", "kernel_cells"):
+            out[f] = list(out[f])
         out["rc"] = rc
         return out
 

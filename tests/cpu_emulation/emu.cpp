@@ -35,7 +35,7 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   a.counts = counts;
   a.compact = compact;
   a.counters = counters;
-  const hipError_t e = run_react(m, a, nullptr, 3 * 20000 + 100, trips);
+  const hipError_t e = run_react(m, a, nullptr, 3 * 20000 + 100, trips, nullptr);
   delete[] ws;
   delete[] iws;
   delete[] lists;

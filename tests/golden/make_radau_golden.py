@@ -1,0 +1,54 @@
+"""Reactor-advance fixtures from an independent stiff integrator.
+
+SciPy's Radau IIA (order 5, scipy.integrate.solve_ivp) at rtol 1e-12 integrates the
+reactor right-hand side (the oracle's, T re-solved from the enthalpy) for seeded cells
+of every workload; the BDF restatements are checked against these end states.  Writes
+radau_<workload>.json.  Run: python tests/golden/make_radau_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+from scipy.integrate import solve_ivp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from oracle import oracle as O  # noqa: E402
+from pelelm_amd import cells  # noqa: E402
+
+
+def advance(m, y0, rE0, dt):
+    def f(t, y):
+        return m.rhs(t, y, rhoH0=rE0)
+
+    def jac(t, y):
+        yy = y.copy()
+        rho = yy[:-1].sum()
+        yy[-1], _ = m.hy2t((rE0) / rho, yy[:-1] / rho, yy[-1])
+        return m.jac(yy)
+
+    sol = solve_ivp(f, (0.0, dt), y0, method="Radau", rtol=1e-12, atol=1e-20, jac=jac)
+    assert sol.success, sol.message
+    return sol.y[:, -1]
+
+
+def main():
+    for wl, ncell in (("lidryer_16", 8), ("drm19_64", 8), ("drm19_128_mix", 8)):
+        d = cells.generate_workload(wl, ncell=64)
+        m = O.Mech(d["mech"])
+        # a spread of progress variables: every 8th generated cell
+        idx = list(range(0, 64, 8))[:ncell]
+        out = []
+        for c in idx:
+            y0 = np.r_[d["rY"][:, c], d["T"][c]]
+            yend = advance(m, y0, d["rE"][c], d["dt"])
+            out.append({"cell": c, "y0": y0.tolist(), "rE0": float(d["rE"][c]), "y_end": yend.tolist()})
+        with open(os.path.join(HERE, f"radau_{wl}.json"), "w") as fh:
+            json.dump({"workload": wl, "mechanism": d["mech"], "dt": d["dt"], "rtol": 1e-12, "atol": 1e-20,
+                       "cells": out}, fh)
+        print(wl, "done")
+
+
+if __name__ == "__main__":
+    main()

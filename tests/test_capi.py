@@ -1,0 +1,82 @@
+"""The C ABI: every symbol include/pelelm_chem.h declares is exported, and the
+host-side argument checking behaves (no GPU call is made by these tests)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from pelelm_amd import reactor
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "pelelm_chem.h")).read()
+    return sorted(set(re.findall(r"\b(mi_chem_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = reactor.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 11
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(reactor.EXPORTS) == syms
+
+
+def test_create_rejects_unknown_integrator():
+    with pytest.raises(reactor.ReactorError):
+        reactor.ReactorBase.create("ReactorRK64")
+
+
+@pytest.mark.parametrize("name", ["ReactorCvode", "ReactorBDF", "ReactorNull"])
+def test_create_init_close(name):
+    r = reactor.ReactorBase.create(name)
+    r.init(2, 100)
+    r.close()
+
+
+def test_init_rejects_e_reactor():
+    r = reactor.ReactorBase.create("ReactorCvode")
+    with pytest.raises(reactor.ReactorError, match="reactor_type"):
+        r.init(1, 1)
+    r.close()
+
+
+@pytest.mark.parametrize("key,value", [("ode.rtol", "-1"), ("ode.rtol", "abc"), ("cvode.max_order", "7"),
+                                       ("cvode.solve_type", "GMRES"), ("mi.mechanism", "gri30"),
+                                       ("mi.schedule", "fifo"), ("no.such.key", "1")])
+def test_set_rejects_bad_keys(key, value):
+    r = reactor.ReactorBase.create("ReactorCvode")
+    with pytest.raises(reactor.ReactorError):
+        r.set(key, value)
+    r.close()
+
+
+def test_set_accepts_reference_deck_keys():
+    r = reactor.ReactorBase.create("ReactorCvode")
+    for k, v in [("ode.rtol", "1.0e-6"), ("ode.atol", "1.0e-5"), ("cvode.solve_type", "denseAJ_direct"),
+                 ("cvode.max_order", "4"), ("peleLM.use_typ_vals_chem", "1"), ("mi.mechanism", "LiDryer"),
+                 ("mi.schedule", "plain"), ("mi.schedule", "queue")]:
+        r.set(k, v)
+    assert r.nspec == 9
+    r.set_typ_vals_ode([1e-3] * 9 + [1500.0])
+    with pytest.raises(reactor.ReactorError):
+        r.set_typ_vals_ode([1e-3] * 9)
+    r.close()
+
+
+def test_react_before_init_is_a_state_error():
+    lib = reactor.lib()
+    h = C.c_void_p()
+    assert lib.mi_chem_create(b"ReactorCvode", C.byref(h)) == 0
+    dt, tm = C.c_double(1e-6), C.c_double(0.0)
+    rc = lib.mi_chem_react_flat(h, None, None, None, None, None, None, None, 0, C.byref(dt), C.byref(tm), None)
+    assert rc == reactor.MI_ESTATE
+    assert b"before init" in lib.mi_chem_last_error(h)
+    assert lib.mi_chem_init(h, 2, 1) == 0
+    dt.value = -1.0
+    rc = lib.mi_chem_react_flat(h, None, None, None, None, None, None, None, 0, C.byref(dt), C.byref(tm), None)
+    assert rc == reactor.MI_EINVAL
+    assert lib.mi_chem_close(h) == 0

@@ -66,9 +66,9 @@ def run_jac_lu(L, mech, y, T, fT, gamma, b):
 
 
 def _dump(name, **arrays):
-    """Keep the compared arrays for inspection when a GPU run writes gpurun_out/."""
-    out = os.path.join(os.path.dirname(HERE), "gpurun_out")
-    if os.path.isdir(out):
+    """Keep the compared arrays for inspection when PELE_UNIT_DUMP names a directory."""
+    out = os.environ.get("PELE_UNIT_DUMP")
+    if out and os.path.isdir(out):
         np.savez(os.path.join(out, f"unit_{name}.npz"), **arrays)
 
 

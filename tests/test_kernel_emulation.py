@@ -1,0 +1,58 @@
+"""The reactor's device code, run serially on the CPU, against the oracle.
+
+tests/cpu_emulation builds reactor_bdf.hip with the host compiler through a small
+stand-in for the HIP runtime (one lane at a time), so the trip pipeline -- k_init,
+k_eval, k_setup, k_consume and the host loop -- is checked on every CPU test run.
+The device code follows the same algorithm as the oracle, so every cell must take the
+same number of right-hand-side evaluations and agree to rounding.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pelelm_amd import cells
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU = os.path.join(HERE, "cpu_emulation")
+LIB = os.path.join(EMU, "_build", "libemu.so")
+SRC = [os.path.join(EMU, "emu.cpp"), os.path.join(EMU, "hip", "hip_runtime.h"),
+       os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "reactor_bdf.hip"),
+       os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "reactor_internal.h")]
+
+
+@pytest.fixture(scope="module")
+def emu():
+    stale = not os.path.exists(LIB) or any(os.path.getmtime(s) > os.path.getmtime(LIB) for s in SRC)
+    if stale:
+        os.makedirs(os.path.dirname(LIB), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + EMU, "-o", LIB, SRC[0]], check=True)
+    L = C.CDLL(LIB)
+    L.emu_react.restype = C.c_int
+    return L
+
+
+@pytest.mark.parametrize("wl,mech,n", [("lidryer_16", "LiDryer", 256), ("drm19_64", "drm19", 256),
+                                       ("drm19_128_mix", "drm19", 256)])
+@pytest.mark.parametrize("compact", [1, 0])
+def test_emulated_kernels_match_oracle(emu, wl, mech, n, compact):
+    d = cells.generate_workload(wl, ncell=n)
+    m = O.Mech(mech)
+    ref = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10, max_steps=20000)
+    rY, T, rE = d["rY"].copy(), d["T"].copy(), d["rE"].copy()
+    FC = np.zeros(n)
+    cnt = np.zeros(16, dtype=np.uint64)
+    trips = C.c_long(0)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    rc = emu.emu_react(mech.encode(), C.c_long(n), p(rY), p(T), p(rE), p(FC), C.c_double(d["dt"]),
+                       C.c_double(1e-6), C.c_double(1e-10), 4, compact, p(cnt), C.byref(trips))
+    assert rc == 0
+    assert cnt[0] == n and cnt[8] == 0          # cells integrated, none failed
+    np.testing.assert_array_equal(FC, ref["FC"])  # identical step sequences
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    assert np.max(np.abs(rY - ref["rY"]) / tol) < 1e-3
+    assert np.max(np.abs(T - ref["T"])) < 1e-8
+    assert trips.value >= int(ref["FC"].max())

@@ -1,0 +1,116 @@
+"""Mechanism data pins: species order, thermo consistency, reference-held PMF fixtures.
+
+The PMF profiles in the reference (Exec/RegTests/FlameSheet/drm19_pmf.dat,
+Exec/RegTests/EB_FlamePastCylinder/pmf_040_01Atm.dat) are the only numbers the reference
+holds for this path; their columns are mole fractions.  They pin molecular weights and
+the gas constant (every row obeys the ideal-gas law) and the enthalpies (a 1-D flame's
+far fields have equal mixture enthalpy).  Those two profiles are copied here as small
+fixtures (first and last rows only) so the tests run without the reference tree.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.optimize import brentq
+
+from pelelm_amd.mechanisms import mechanism as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# species order of the PMF headers (drm19_pmf.dat:1, pmf_040_01Atm.dat:1)
+PMF_ORDER = {
+    "drm19": "H2 H O O2 OH H2O HO2 CH2 CH2(S) CH3 CH4 CO CO2 HCO CH2O CH3O C2H4 C2H5 C2H6 N2 AR".split(),
+    "LiDryer": "H2 O2 H2O H O OH HO2 H2O2 N2".split(),
+}
+
+# heats of formation at 298.15 K [kJ/mol] (Active Thermochemical Tables / JANAF) and
+# the tolerance each is held to
+DHF = {"H2": (0.0, 0.5), "O2": (0.0, 0.5), "N2": (0.0, 0.5), "AR": (0.0, 0.5), "H": (218.0, 1.0),
+       "O": (249.2, 1.0), "OH": (38.0, 1.5), "H2O": (-241.8, 1.0), "HO2": (12.3, 1.5), "H2O2": (-135.9, 1.0),
+       "CH4": (-74.6, 1.0), "CO": (-110.5, 1.0), "CO2": (-393.5, 1.0), "CH3": (146.6, 1.5), "CH2": (391.2, 2.0),
+       "CH2(S)": (429.0, 2.0), "HCO": (41.6, 1.5), "CH2O": (-108.9, 1.5), "CH3O": (17.0, 6.0),
+       "C2H4": (52.4, 1.0), "C2H5": (120.0, 2.0), "C2H6": (-84.0, 1.0)}
+
+
+def fixture_rows(name):
+    return np.load(os.path.join(HERE, "golden", f"pmf_endpoints_{name}.npz"))
+
+
+@pytest.mark.parametrize("name,nr", [("drm19", 84), ("LiDryer", 21)])
+def test_load_and_order(name, nr):
+    m = M.load(name)
+    assert m.names == PMF_ORDER[name]
+    assert m.nreac == nr
+
+
+@pytest.mark.parametrize("name", ["drm19", "LiDryer"])
+def test_nasa_continuity_at_break(name):
+    m = M.load(name)
+    for sp in m.species:
+        Tm = sp.T[1]
+        lo, hi = sp.low, sp.high
+
+        def ev(a, T):
+            cp = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])))
+            h = a[0] + T * (a[1] / 2 + T * (a[2] / 3 + T * (a[3] / 4 + T * a[4] / 5))) + a[5] / T
+            s = a[0] * math.log(T) + T * (a[1] + T * (a[2] / 2 + T * (a[3] / 3 + T * a[4] / 4))) + a[6]
+            return np.array([cp, h, s])
+        d = np.abs(ev(lo, Tm) - ev(hi, Tm))
+        assert d.max() < 5e-5, (sp.name, d)
+
+
+@pytest.mark.parametrize("name", ["drm19", "LiDryer"])
+def test_heats_of_formation(name):
+    m = M.load(name)
+    for sp in m.species:
+        ref, tol = DHF[sp.name]
+        dhf = M.h_RT(sp, 298.15) * 298.15 * M.R_CGS * 1e-10   # erg/mol -> kJ/mol
+        assert abs(dhf - ref) < tol, (sp.name, dhf, ref)
+
+
+@pytest.mark.parametrize("name", ["drm19", "LiDryer"])
+def test_cp_is_dh_dT(name):
+    m = M.load(name)
+    for sp in m.species:
+        for T in (400.0, 900.0, 1500.0, 2500.0):
+            e = 1e-3
+            dh = (M.h_RT(sp, T + e) * (T + e) - M.h_RT(sp, T - e) * (T - e)) / (2 * e)
+            assert abs(dh - M.cp_R(sp, T)) < 1e-6 * M.cp_R(sp, T)
+
+
+@pytest.mark.parametrize("name", ["drm19", "LiDryer"])
+def test_pmf_ideal_gas_density(name):
+    m = M.load(name)
+    d = fixture_rows(name)
+    W = np.array(m.W)
+    for row in (d["first"], d["last"]):
+        T, rho, X = row[1], row[3], np.maximum(row[4:], 0.0)
+        Wbar = np.dot(X, W) / X.sum()
+        # 3e-8 with the CHEMKIN gas constant; the CODATA-2018 value would give 5.7e-6
+        assert abs(M.P_ATM * Wbar / (M.R_CGS * T) / rho - 1.0) < 1e-7
+
+
+@pytest.mark.parametrize("name,tolK", [("drm19", 1.0), ("LiDryer", 0.05)])
+def test_pmf_far_field_enthalpy_balance(name, tolK):
+    m = M.load(name)
+    d = fixture_rows(name)
+    W = np.array(m.W)
+
+    def hmass(T, X):
+        X = np.maximum(X, 0.0)
+        Y = X * W / np.dot(X, W)
+        return sum(Y[k] * M.h_RT(s, T) * M.R_CGS * T / W[k] for k, s in enumerate(m.species))
+    hin = hmass(d["first"][1], d["first"][4:])
+    Tb = brentq(lambda T: hmass(T, d["last"][4:]) - hin, 500.0, 3500.0)
+    assert abs(Tb - d["last"][1]) < tolK
+
+
+def test_reactions_element_balanced():
+    for name in ("drm19", "LiDryer"):
+        m = M.load(name)
+        for r in m.reactions:
+            for el in m.elements:
+                lhs = sum(m.species[i].composition.get(el, 0) * c for i, c in r.reactants.items())
+                rhs = sum(m.species[i].composition.get(el, 0) * c for i, c in r.products.items())
+                assert lhs == rhs, r.equation

@@ -1,0 +1,183 @@
+"""Pins of the CPU oracle (the checker every GPU test compares against).
+
+Each check is one a plausible mistake in the restatement would fail: the analytic
+Jacobian against complex-step derivatives, wdot against an independent 50-digit
+evaluation and against mass/element conservation and detailed balance, HY2T against
+its defining equation, the BDF against closed-form linear ODEs and against an
+independent stiff integrator (SciPy Radau) on real reactor cells.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from oracle import oracle as O
+from pelelm_amd import cells
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MECHS = ["LiDryer", "drm19"]
+
+
+def _random_state(m, rng, Tlo=300.0, Thi=2800.0):
+    Y = rng.random(m.nspec) ** 3 + 1e-9
+    Y /= Y.sum()
+    T = rng.uniform(Tlo, Thi)
+    rho = 1.01325e6 / np.dot(Y, 1.0 / m.W) / (m.Ru * T)
+    return rho * Y, T
+
+
+@pytest.mark.parametrize("name", MECHS)
+def test_jacobian_matches_complex_step(name):
+    m = O.Mech(name)
+    rng = np.random.default_rng(1)
+    for _ in range(30):
+        rY, T = _random_state(m, rng)
+        y = np.r_[rY, T]
+        fY = rng.normal(size=m.nspec) * 1e-3
+        Ja = m.jac(y, fY=fY, fH=1e5)
+        Jc = m.jac(y, fY=fY, fH=1e5, complex_step=True)
+        scale = np.maximum(np.abs(Jc).max(axis=1, keepdims=True), 1e-300)
+        assert np.max(np.abs(Ja - Jc) / scale) < 1e-12
+
+
+@pytest.mark.parametrize("name", MECHS)
+def test_wdot_matches_mpmath_golden(name):
+    m = O.Mech(name)
+    with open(os.path.join(HERE, "golden", f"wdot_{name}.json")) as f:
+        g = json.load(f)
+    for st in g["states"]:
+        w = m.wdot(np.array(st["C"]), st["T"])
+        ref = np.array(st["wdot"])
+        assert np.max(np.abs(w - ref)) <= 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("name", MECHS)
+def test_wdot_conserves_mass_and_elements(name):
+    m = O.Mech(name)
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        rY, T = _random_state(m, rng)
+        w = m.wdot(rY / m.W, T)
+        scale = np.abs(w * m.W).max()
+        assert abs(np.dot(w, m.W)) <= 1e-13 * scale
+        assert np.max(np.abs(m.ecomp.T.astype(float) @ w)) <= 1e-13 * np.abs(w).max() * m.ecomp.max()
+
+
+@pytest.mark.parametrize("name,fuel,o2", [("LiDryer", "H2", 0.5), ("drm19", "CH4", 2.0)])
+@pytest.mark.parametrize("phi", [0.6, 1.0, 1.4])
+def test_detailed_balance_at_hp_equilibrium(name, fuel, o2, phi):
+    """At the HP-equilibrium state (an independent element-potential solver) every
+    reversible reaction's forward and reverse rates of progress must coincide."""
+    m = O.Mech(name)
+    names = cells.species_names(name)
+    X = np.zeros(m.nspec)
+    X[names.index(fuel)] = phi
+    X[names.index("O2")] = o2
+    X[names.index("N2")] = o2 * 0.79 / 0.21
+    X /= X.sum()
+    Y, T = cells.equilibrium_hp(name, X)
+    rho = 1.01325e6 / np.dot(Y, 1.0 / m.W) / (m.Ru * T)
+    qf, qr = m.rop(rho * Y / m.W, T)
+    rel = np.abs(qf - qr) / np.maximum(np.maximum(qf, qr), 1e-300)
+    assert rel.max() < 1e-10
+
+
+def test_single_reaction_arrhenius():
+    """H + O2 <=> O + OH (LiDryer reaction 0): the forward rate of progress is the
+    textbook modified-Arrhenius product k_f [H][O2]."""
+    m = O.Mech("LiDryer")
+    names = cells.species_names("LiDryer")
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        rY, T = _random_state(m, rng)
+        C = rY / m.W
+        qf, _ = m.rop(C, T)
+        kf = 3.547e15 * T ** -0.406 * np.exp(-16599.0 / (8.314510e7 / 4.184e7) / T)
+        assert qf[0] == pytest.approx(kf * C[names.index("H")] * C[names.index("O2")], rel=1e-13)
+
+
+@pytest.mark.parametrize("name", MECHS)
+def test_hy2t_inverts_enthalpy(name):
+    m = O.Mech(name)
+    rng = np.random.default_rng(4)
+    for _ in range(40):
+        Y = rng.random(m.nspec) + 1e-9
+        Y /= Y.sum()
+        T = rng.uniform(250.0, 3000.0)
+        _, hRT, _ = m.thermo(T)
+        h = np.dot(Y, hRT / m.W) * m.Ru * T
+        Tg, it = m.hy2t(h, Y, T + rng.uniform(-200, 200))
+        assert Tg == pytest.approx(T, rel=1e-12)
+        assert it < 10
+
+
+@pytest.mark.parametrize("order,rtol,bound", [(1, 1e-5, 2e-3), (2, 1e-6, 5e-5), (4, 1e-8, 2e-7), (5, 1e-8, 2e-7)])
+def test_bdf_linear_closed_form(order, rtol, bound):
+    rng = np.random.default_rng(1)
+    n = 6
+    Q = np.linalg.qr(rng.normal(size=(n, n)))[0]
+    A = Q @ np.diag(-np.logspace(0, 4, n)) @ Q.T      # stiff, symmetric negative definite
+    y0 = rng.normal(size=n)
+    y, r, st = O.bdf_linear(A, y0, 1.0, rtol=rtol, atol=1e-12, max_order=order)
+    assert r == 0
+    exact = sl.expm(A) @ y0
+    assert np.max(np.abs(y - exact)) < bound * np.max(np.abs(exact))
+    assert st.qlast <= order
+
+
+def test_bdf_error_scales_with_tolerance():
+    A = np.array([[-1.0, 1.0], [0.0, -50.0]])
+    y0 = np.array([1.0, 1.0])
+    exact = sl.expm(A * 2.0) @ y0
+    errs = []
+    for rtol in (1e-4, 1e-6, 1e-8):
+        y, r, _ = O.bdf_linear(A, y0, 2.0, rtol=rtol, atol=rtol * 1e-3)
+        errs.append(abs(y[0] - exact[0]) / abs(exact[0]))   # y[1] ~ e^-100 is below atol
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 1e-6
+
+
+@pytest.mark.parametrize("wl", ["lidryer_16", "drm19_64", "drm19_128_mix"])
+def test_bdf_matches_radau_fixture(wl):
+    with open(os.path.join(HERE, "golden", f"radau_{wl}.json")) as f:
+        g = json.load(f)
+    m = O.Mech(g["mechanism"])
+    N = m.nspec
+    y0 = np.array([c["y0"] for c in g["cells"]]).T
+    rE0 = np.array([c["rE0"] for c in g["cells"]])
+    ref = np.array([c["y_end"] for c in g["cells"]]).T
+    tight = m.react(y0[:N], y0[N], rE0, g["dt"], rtol=1e-11, atol=1e-20)
+    assert tight["nfail"] == 0
+    np.testing.assert_allclose(tight["T"], ref[N], rtol=1e-9)
+    assert np.max(np.abs(tight["rY"] - ref[:N]) / (1e-8 * np.abs(ref[:N]) + 1e-18)) < 1.0
+    loose = m.react(y0[:N], y0[N], rE0, g["dt"], rtol=1e-6, atol=1e-10)
+    tol = 1e-6 * np.abs(ref[:N]) + 1e-10
+    assert np.max(np.abs(loose["rY"] - ref[:N]) / tol) < 20.0
+    # rtol 1e-6 over dt = 1e-5 s leaves up to ~3e-6 relative global error in T
+    assert np.max(np.abs(loose["T"] - ref[N]) / ref[N]) < 1e-5
+
+
+def test_react_conserves_mass_and_energy_with_forcing():
+    d = cells.generate_workload("drm19_64", ncell=256)
+    m = O.Mech("drm19")
+    rng = np.random.default_rng(5)
+    fY = rng.normal(size=d["rY"].shape) * 1e-3 * d["rY"]
+    fY -= fY.mean(axis=0, keepdims=True) * 0.0
+    fH = rng.normal(size=256) * 1e6
+    out = m.react(d["rY"], d["T"], d["rE"], d["dt"], rYsrc=fY, rEsrc=fH)
+    assert out["nfail"] == 0
+    np.testing.assert_allclose(out["rY"].sum(0), d["rY"].sum(0) + d["dt"] * fY.sum(0), rtol=1e-12)
+    np.testing.assert_allclose(out["rE"], d["rE"] + d["dt"] * fH, rtol=1e-15)
+
+
+def test_react_mask_skips_cells():
+    d = cells.generate_workload("lidryer_16", ncell=64)
+    m = O.Mech("LiDryer")
+    mask = np.ones(64, dtype=np.int32)
+    mask[::3] = -1
+    out = m.react(d["rY"], d["T"], d["rE"], d["dt"], mask=mask)
+    np.testing.assert_array_equal(out["rY"][:, ::3], d["rY"][:, ::3])
+    np.testing.assert_array_equal(out["FC"][::3], 0.0)
+    assert np.all(out["FC"][1::3] > 0)
