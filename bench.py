@@ -113,8 +113,9 @@ def main(argv=None):
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the reactor's cells are independent and shard across GPUs "
-              f"without communication (see DESIGN.md); --gpus {args.gpus} is not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU: the reactor's cells are independent, so a multi-GPU run is "
+              f"N copies of this one on disjoint boxes with no communication; --gpus {args.gpus} is not run.",
+              file=sys.stderr)
         return 2
 
     import torch
@@ -139,7 +140,7 @@ def main(argv=None):
     # roofline of the dominant kernel (the right-hand-side kernel when profiled)
     roofline = None
     if not args.no_profile:
-        kinds = ["rhs (k_eval)", "jacobian+LU (k_setup)", "newton/step (k_consume)", "other"]
+        kinds = ["rhs (k_eval)", "jacobian (k_setup)", "newton/step (k_consume)", "LU (k_lu_coop)"]
         kms = st["kernel_ms"]
         dom = int(np.argmax(kms))
         fixed, per_iter, trans = rhs_ops("drm19")

@@ -62,7 +62,7 @@ typedef struct {
   long ntrips;          /* right-hand-side trips the call took                  */
   long nhy2t_iters;     /* Newton iterations of the T-from-enthalpy solves      */
   /* with mi.profile = 1: device time (ms), launches and cells processed per kernel
-     kind [0] right-hand side, [1] Jacobian/LU set-up, [2] Newton/step logic, [3] other */
+     kind [0] right-hand side, [1] Jacobian set-up, [2] Newton/step logic, [3] LU */
   double kernel_ms[4];
   long kernel_launches[4];
   long kernel_cells[4];
@@ -83,7 +83,8 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
      ode.rtol, ode.atol, cvode.max_order, cvode.max_num_steps (default 20000),
      cvode.solve_type (denseAJ_direct), peleLM.use_typ_vals_chem (0/1),
      mi.mechanism (drm19 | LiDryer), mi.schedule (queue: finished cells leave the work
-     lists, default | plain: fixed cell-to-lane mapping), mi.lu (fp64), mi.ir_sweeps,
+     lists, default | plain: fixed cell-to-lane mapping), mi.lu (fp64: 32 lanes
+     factorise one cell's matrix, default | fp64_lane: one lane per cell), mi.ir_sweeps,
      mi.on_fail (continue | abort), mi.profile (0 | 1: per-kernel event timing). */
 int mi_chem_set(void* reactor, const char* key, const char* value);
 

@@ -261,7 +261,8 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
     else return fail(r, MI_EINVAL, "mi.schedule must be plain or queue");
   } else if (k == "mi.lu") {
     if (v == "fp64") r->lu_mode = LU_FP64;
-    else return fail(r, MI_EINVAL, "mi.lu must be fp64");
+    else if (v == "fp64_lane") r->lu_mode = LU_FP64_LANE;
+    else return fail(r, MI_EINVAL, "mi.lu must be fp64 or fp64_lane");
   } else if (k == "mi.ir_sweeps") {
     if (!parse_long(value, &l) || l < 0 || l > 10) return fail(r, MI_EINVAL, "mi.ir_sweeps must be 0..10");
     r->ir_sweeps = (int)l;

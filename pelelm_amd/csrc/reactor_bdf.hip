@@ -23,6 +23,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstddef>
 
 #include "reactor_internal.h"
 
@@ -67,10 +68,11 @@ struct Slots {
   static constexpr int Y = ATOL + N1;                 // evaluation point of this trip
   static constexpr int F = Y + N1;                    // right-hand side at Y
   static constexpr int TS = F + N1;                   // temperature re-solved at Y
-  static constexpr int LANE = TS + 1;                 // integrator state words
+  static constexpr int SING = TS + 1;                 // 0, or 1 + first zero-pivot column of the last LU
+  static constexpr int LANE = SING + 1;               // integrator state words
   static constexpr int LANE_WORDS = 96;
   static constexpr int TOTAL = LANE + LANE_WORDS;
-  static constexpr int IPIV = 0;                      // int slots
+  static constexpr int PERM = 0;                      // int slots: row i of LU is row PERM[i] of M
   static constexpr int ITOTAL = N1;
 };
 
@@ -193,6 +195,7 @@ __device__ __noinline__ int lu_factor(const WS& ws, const IWS& iws, const double
   using S = Slots<N1>;
 #pragma unroll 1
   for (int i = 0; i < N1; ++i) {
+    iws[S::PERM + i] = i;
 #pragma unroll
     for (int j = 0; j < N1; ++j) ws[S::LU + i * N1 + j] = (i == j ? 1.0 : 0.0) - gamma * ws[S::JS + i * N1 + j];
   }
@@ -206,14 +209,16 @@ __device__ __noinline__ int lu_factor(const WS& ws, const IWS& iws, const double
       const double a = fabs(ws[S::LU + i * N1 + k]);
       if (a > amax) { amax = a; p = i; }
     }
-    iws[S::IPIV + k] = p;
     if (amax == 0.0) { if (!sing) sing = k + 1; continue; }
     if (p != k) {
+      const int t = iws[S::PERM + k];
+      iws[S::PERM + k] = iws[S::PERM + p];
+      iws[S::PERM + p] = t;
 #pragma unroll 1
       for (int j = 0; j < N1; ++j) {
-        const double t = ws[S::LU + k * N1 + j];
+        const double t2 = ws[S::LU + k * N1 + j];
         ws[S::LU + k * N1 + j] = ws[S::LU + p * N1 + j];
-        ws[S::LU + p * N1 + j] = t;
+        ws[S::LU + p * N1 + j] = t2;
       }
     }
     const double inv = 1.0 / ws[S::LU + k * N1 + k];
@@ -230,21 +235,13 @@ __device__ __noinline__ int lu_factor(const WS& ws, const IWS& iws, const double
   return sing;
 }
 
-// solve (I - gamma J) x = b with b given in ws[TMP]; result in x (registers)
+// solve (I - gamma J) x = b with b given in ws[TMP]; result in x (registers).
+// P M = L U with row i of the factors holding row PERM[i] of M.
 template <int N1>
 __device__ __forceinline__ void lu_solve(const WS& ws, const IWS& iws, double* __restrict__ x) {
   using S = Slots<N1>;
-#pragma unroll 1
-  for (int k = 0; k < N1; ++k) {
-    const int p = iws[S::IPIV + k];
-    if (p != k) {
-      const double t = ws[S::TMP + k];
-      ws[S::TMP + k] = ws[S::TMP + p];
-      ws[S::TMP + p] = t;
-    }
-  }
 #pragma unroll
-  for (int i = 0; i < N1; ++i) x[i] = ws[S::TMP + i];
+  for (int i = 0; i < N1; ++i) x[i] = ws[S::TMP + iws[S::PERM + i]];
 #pragma unroll
   for (int k = 0; k < N1; ++k)
 #pragma unroll
@@ -287,9 +284,10 @@ struct Lane {
   // evaluation request for the next trip: y = zn0 + a1 * zn1 + a2 * acor at t = teval
   double a1, a2, teval;
   int phase, status;
-  // cell constants: (rho h)_0 and its forcing; hold = 1 skips one consume pass
+  // cell constants: (rho h)_0 and its forcing; setup_pending = 1 when this trip's
+  // evaluation started an iteration-matrix set-up whose factorisation is to be consumed
   double rE0, fH;
-  int hold;
+  int setup_pending;
 
   __device__ __forceinline__ double& zn(int j, int i) const { return ws[S::ZN + j * N1 + i]; }
 
@@ -665,8 +663,9 @@ struct Lane {
 
   __device__ bool needs_setup() const { return phase == PH_NEWTON && m == 0 && callSetup; }
 
-  // iteration-matrix set-up at the first evaluation of a Newton attempt: Jacobian
-  // when stale (or after a convergence failure), then LU of I - gamma J
+  // iteration-matrix set-up at the first evaluation of a Newton attempt: the Jacobian
+  // when stale (or after a convergence failure) and the set-up bookkeeping; the LU of
+  // I - gamma J follows in its own kernel
   __device__ void setup(const double* y, const double* f, const double Tsol) {
     const double dgamma = fabs(gamma / gammap - 1.0);
     const bool jbad = (nst == 0) || (nst >= nstlj + MSBJ) ||
@@ -679,16 +678,21 @@ struct Lane {
     } else {
       jcur = false;
     }
-    const int sing = lu_factor<N1>(ws, iws, gamma);
     ++nsetups;
-    gamrat = 1.0;
-    gammap = gamma;
-    crate = 1.0;
+    gammap = gamma;          // gamrat = 1 and crate = 1 once the factorisation is in place
     nstlp = nst;
+    setup_pending = 1;
+  }
+  // after the factorisation: false when it was singular (the attempt is abandoned)
+  __device__ bool after_setup(int sing) {
+    setup_pending = 0;
+    gamrat = 1.0;
+    crate = 1.0;
     if (sing) {
       conv_failure();
-      hold = 1;   // the evaluation of this trip belongs to the abandoned attempt
+      return false;
     }
+    return true;
   }
 
   // one Newton iteration with f = f(tn, zn0 + acor); returns true when tout is reached
@@ -757,7 +761,7 @@ struct Lane {
     ewtset();
     status = ST_OK;
     phase = PH_INIT_F0;
-    hold = 0;
+    setup_pending = 0;
     a1 = 0.0; a2 = 0.0; teval = 0.0;
   }
 };
@@ -925,7 +929,7 @@ __global__ void __launch_bounds__(64) k_setup(const ReactArgs a, const int* __re
   const WS w = cell_ws<M>(a.ws, c);
   Lane<M> st;
   load_lane(st, w, cell_iws<M>(a.iws, c), a);
-  if (!st.needs_setup()) return;
+  if (!st.needs_setup() || st.setup_pending) return;
   double y[N1], f[N1];
 #pragma unroll
   for (int i = 0; i < N1; ++i) {
@@ -934,6 +938,111 @@ __global__ void __launch_bounds__(64) k_setup(const ReactArgs a, const int* __re
   }
   st.setup(y, f, w[S::TS]);
   store_lane(st, w);
+}
+
+// LU of M = I - gamma J for every cell whose set-up is pending, one lane per cell
+// (used where lanes cannot cooperate, e.g. when the kernels run serially on a CPU)
+template <class M>
+__global__ void __launch_bounds__(64) k_lu_lane(const ReactArgs a, const int* __restrict__ list,
+                                                const unsigned int* __restrict__ count) {
+  using S = Slots<M::N1>;
+  const long idx = (long)blockIdx.x * 64 + threadIdx.x;
+  if (idx >= (long)*count) return;
+  const long c = list[idx];
+  const WS w = cell_ws<M>(a.ws, c);
+  Lane<M> st;
+  load_lane(st, w, cell_iws<M>(a.iws, c), a);
+  if (!st.setup_pending) return;
+  w[S::SING] = (double)lu_factor<M::N1>(w, st.iws, st.gamma);
+}
+
+// Cooperative LU of M = I - gamma J: 32 lanes per cell (row r of M in lane r), two
+// cells per wave.  Column k's pivot is the largest |m_rk| over the rows not yet
+// pivoted (a 32-lane max-reduction); its row is broadcast through LDS and every
+// other remaining row is eliminated in registers.  Row r ends as row `step` of the
+// factors, where `step` is the column at which it was chosen (PERM[step] = r), which
+// is partial pivoting with the interchanges kept as a permutation.
+template <class M>
+__global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __restrict__ list,
+                                                const unsigned int* __restrict__ count) {
+  constexpr int N1 = M::N1;
+  static_assert(N1 <= 32, "the cooperative LU holds one row per lane of a 32-lane half-wave");
+  using S = Slots<N1>;
+  constexpr int GAMMA_WORD = (int)(offsetof(Lane<M>, gamma) / 8);
+  constexpr int PEND_WORD = (int)(offsetof(Lane<M>, setup_pending) / 8);
+  __shared__ double prow[2][N1];
+  const int lane = threadIdx.x;
+  const int half = lane >> 5, r = lane & 31;
+  const long idx = (long)blockIdx.x * 2 + half;
+  const long n = (long)*count;
+  long c = -1;
+  bool work = false;
+  double gamma = 0.0;
+  if (idx < n) {
+    c = list[idx];
+    const WS w = cell_ws<M>(a.ws, c);
+    int pend;
+    double pw = w[S::LANE + PEND_WORD];
+    __builtin_memcpy(&pend, reinterpret_cast<const char*>(&pw) + (offsetof(Lane<M>, setup_pending) % 8),
+                     sizeof(int));
+    work = (pend != 0);
+    gamma = w[S::LANE + GAMMA_WORD];
+  }
+  const bool row = work && r < N1;
+  double m[N1];
+  if (row) {
+    const WS w = cell_ws<M>(a.ws, c);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) m[j] = (r == j ? 1.0 : 0.0) - gamma * w[S::JS + r * N1 + j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < N1; ++j) m[j] = 0.0;
+  }
+  bool done = !row;
+  int my_step = 0, sing = 0;
+#pragma unroll
+  for (int k = 0; k < N1; ++k) {
+    double v = done ? -1.0 : fabs(m[k]);
+    if (v != v) v = HUGE_VAL;   // a NaN row still takes a definite place in the order
+    int who = r;
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(v, off, 32);
+      const int ow = __shfl_xor(who, off, 32);
+      if (ov > v || (ov == v && ow < who)) {
+        v = ov;
+        who = ow;
+      }
+    }
+    if (v <= 0.0 && sing == 0) sing = k + 1;
+    if (row && r == who) {
+#pragma unroll
+      for (int j = 0; j < N1; ++j) prow[half][j] = m[j];
+    }
+    __syncthreads();
+    if (!done && r != who && v > 0.0) {   // v: this column's pivot magnitude, equal in all 32 lanes
+      const double inv = 1.0 / prow[half][k];
+      const double l = m[k] * inv;
+      m[k] = l;
+      if (l != 0.0) {
+#pragma unroll
+        for (int j = k + 1; j < N1; ++j) m[j] -= l * prow[half][j];
+      }
+    }
+    if (r == who && !done) {
+      done = true;
+      my_step = k;
+    }
+    __syncthreads();
+  }
+  if (row) {
+    const WS w = cell_ws<M>(a.ws, c);
+    const IWS iw = cell_iws<M>(a.iws, c);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) w[S::LU + my_step * N1 + j] = m[j];
+    iw[S::PERM + my_step] = r;
+    if (r == 0) w[S::SING] = (double)sing;
+  }
 }
 
 // advance every active cell's state machine by this trip's evaluation
@@ -960,9 +1069,9 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
 #pragma unroll
       for (int i = 0; i < N1; ++i) f[i] = w[S::F + i];
       bool reached = false;
-      if (st.hold) {
-        st.hold = 0;
-      } else {
+      bool proceed = true;
+      if (st.setup_pending) proceed = st.after_setup((int)w[S::SING]);
+      if (proceed) {
         switch (st.phase) {
           case PH_INIT_F0:
 #pragma unroll
@@ -1013,7 +1122,7 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
         t_ncfn = st.ncfn; t_netf = st.netf; t_fail = ok ? 0 : 1; t_maxst = st.nst;
       } else {
         keep = true;
-        want_setup = st.needs_setup() && !st.hold;
+        want_setup = st.needs_setup();
       }
       store_lane(st, w);
     }
@@ -1145,6 +1254,15 @@ __global__ void null_kernel(const ReactArgs a, int nspec) {
     if (e_ != hipSuccess) return e_;    \
   } while (0)
 
+template <class M>
+hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* count, long n, hipStream_t stream) {
+  if (a.lu_mode == LU_FP64_LANE)
+    hipLaunchKernelGGL(k_lu_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, list, count);
+  else
+    hipLaunchKernelGGL(k_lu_coop<M>, dim3((unsigned)((n + 1) / 2)), dim3(64), 0, stream, a, list, count);
+  return hipGetLastError();
+}
+
 // counts[0..1]: active-cell lists of the current / next trip, counts[2..3]: their set-up
 // lists; without compaction counts[4] is the (fixed) list length and counts[5] the
 // number of cells still integrating.
@@ -1234,6 +1352,9 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
                            &a.counts[2 + cur]);
         PELE_TRY(hipGetLastError());
         PELE_TRY(tm.end(KK_SETUP, nsetup));
+        PELE_TRY(tm.begin());
+        PELE_TRY(launch_lu<M>(a, a.slist[cur], &a.counts[2 + cur], nsetup, stream));
+        PELE_TRY(tm.end(KK_LU, nsetup));
       }
       PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_consume<M>, dim3(ga), dim3(64), 0, stream, a, a.list[cur], &a.counts[cur], a.list[nxt],
@@ -1253,6 +1374,9 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
       hipLaunchKernelGGL(k_setup<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4]);
       PELE_TRY(hipGetLastError());
       PELE_TRY(tm.end(KK_SETUP, 0));
+      PELE_TRY(tm.begin());
+      PELE_TRY(launch_lu<M>(a, a.list[1], &a.counts[4], n, stream));
+      PELE_TRY(tm.end(KK_LU, 0));
       PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_consume<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4], a.list[0],
                          &a.counts[5], a.slist[0], &a.counts[6]);

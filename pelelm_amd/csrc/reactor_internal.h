@@ -24,7 +24,9 @@ struct IntView {
 };
 
 enum Schedule : int { SCHED_PLAIN = 0, SCHED_COMPACT = 1 };
-enum LuMode : int { LU_FP64 = 0, LU_FP32_IR = 1 };
+// LU_FP64: cooperative FP64 factorisation (32 lanes per cell); LU_FP64_LANE: one lane
+// per cell (the form that runs when the kernels execute serially on a CPU)
+enum LuMode : int { LU_FP64 = 0, LU_FP32_IR = 1, LU_FP64_LANE = 2 };
 
 // Per-call options shared by every schedule.
 struct ReactArgs {
@@ -73,7 +75,7 @@ long iws_slots(int mech);       // ints per lane
 // Optional per-kernel timing: when `prof` is non-null every launch of the trip
 // kernels is bracketed by events and the per-kind device time (ms), launch count and
 // cells processed are accumulated into prof[kind].
-enum KernelKind : int { KK_EVAL = 0, KK_SETUP, KK_CONSUME, KK_OTHER, NKK };
+enum KernelKind : int { KK_EVAL = 0, KK_SETUP, KK_CONSUME, KK_LU, NKK };
 struct KernelProf {
   double ms[NKK];
   long launches[NKK];

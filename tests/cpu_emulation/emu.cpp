@@ -34,6 +34,7 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   }
   a.counts = counts;
   a.compact = compact;
+  a.lu_mode = LU_FP64_LANE;   // lanes run one at a time here, so they cannot cooperate
   a.counters = counters;
   const hipError_t e = run_react(m, a, nullptr, 3 * 20000 + 100, trips, nullptr);
   delete[] ws;

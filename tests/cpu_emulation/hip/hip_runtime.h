@@ -45,6 +45,9 @@ template <class T> inline T min(T a, T b) { return a < b ? a : b; }
 inline int __lane_id() { return 0; }   // every emulated lane reduces on its own
 template <class T> inline T __shfl_down(T v, int, int = 64) { return T(0); }
 template <class T> inline T __shfl(T v, int, int = 64) { return v; }
+template <class T> inline T __shfl_xor(T v, int, int = 64) { return v; }
+inline void __syncthreads() {}
+#define __shared__ static
 inline unsigned long long __ballot(int p) { return p ? (1ull << __lane_id()) : 0ull; }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }

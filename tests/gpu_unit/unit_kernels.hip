@@ -276,10 +276,100 @@ int unit_stepctl_t(long n, const double* scal, const double* zn, const double* a
     UNIT_DEV_FREE(p);
   return 0;
 }
+
+// LU of I - gamma J by the reactor's LU kernels (cooperative or one lane per cell) on
+// given matrices.  Cells with pend[c] == 0 have no set-up pending and must be left as
+// they were (LU slots 7.0, permutation -1, singularity word -3).
+template <class M>
+__global__ void k_lu_prep(const double* J, const double* gamma, const int* pend, double* ws, int* iws, long ncell) {
+  constexpr int N1 = M::N1;
+  using S = Slots<N1>;
+  const long c = (long)blockIdx.x * 64 + threadIdx.x;
+  if (c >= ncell) return;
+  const WS w = cell_ws<M>(ws, c);
+  const IWS iw = cell_iws<M>(iws, c);
+  Lane<M> st;
+  __builtin_memset(&st, 0, sizeof(st));
+  st.gamma = gamma[c];
+  st.setup_pending = pend[c];
+  store_lane(st, w);
+  for (int e = 0; e < N1 * N1; ++e) {
+    w[S::JS + e] = J[c * N1 * N1 + e];
+    w[S::LU + e] = 7.0;
+  }
+  for (int i = 0; i < N1; ++i) iw[S::PERM + i] = -1;
+  w[S::SING] = -3.0;
+}
+
+template <class M>
+__global__ void k_lu_read(const double* b, const int* pend, double* x, double* sing, double* lu, int* perm, double* ws,
+                          int* iws, long ncell) {
+  constexpr int N1 = M::N1;
+  using S = Slots<N1>;
+  const long c = (long)blockIdx.x * 64 + threadIdx.x;
+  if (c >= ncell) return;
+  const WS w = cell_ws<M>(ws, c);
+  const IWS iw = cell_iws<M>(iws, c);
+  sing[c] = w[S::SING];
+  for (int e = 0; e < N1 * N1; ++e) lu[c * N1 * N1 + e] = w[S::LU + e];
+  for (int i = 0; i < N1; ++i) perm[c * N1 + i] = iw[S::PERM + i];
+  if (pend[c] && w[S::SING] == 0.0) {
+    for (int i = 0; i < N1; ++i) w[S::TMP + i] = b[c * N1 + i];
+    double xx[N1];
+    lu_solve<N1>(w, iw, xx);
+    for (int i = 0; i < N1; ++i) x[c * N1 + i] = xx[i];
+  }
+}
+
+template <class M>
+int unit_lu_t(int coop, long n, const double* J, const double* gamma, const int* pend, const double* b, double* x,
+              double* sing, double* lu, int* perm) {
+  constexpr int N1 = M::N1;
+#ifdef UNIT_HOST
+  if (coop) return -2;   // lanes of the cooperative kernel cannot run one at a time
+#endif
+  const long nw = (n + 63) / 64;
+  double *dJ = dev_copy(J, n * N1 * N1), *dg = dev_copy(gamma, n), *db = dev_copy(b, n * N1);
+  int* dp = dev_copy(pend, n);
+  double *dx = dev_zero<double>(n * N1), *ds = dev_zero<double>(n), *dlu = dev_zero<double>(n * N1 * N1);
+  int* dperm = dev_zero<int>(n * N1);
+  double* ws = dev_zero<double>(nw * Slots<N1>::TOTAL * 64);
+  int* iws = dev_zero<int>(nw * Slots<N1>::ITOTAL * 64);
+  std::vector<int> ident(n);
+  for (long i = 0; i < n; ++i) ident[i] = (int)i;
+  int* dlist = dev_copy(ident.data(), n);
+  unsigned int hn = (unsigned int)n;
+  unsigned int* dcount = dev_copy(&hn, 1);
+  ReactArgs a;
+  __builtin_memset(&a, 0, sizeof(a));
+  a.ws = ws;
+  a.iws = iws;
+  a.lu_mode = coop ? LU_FP64 : LU_FP64_LANE;
+  hipLaunchKernelGGL(k_lu_prep<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, dJ, dg, dp, ws, iws, n);
+  (void)launch_lu<M>(a, dlist, dcount, n, nullptr);
+  hipLaunchKernelGGL(k_lu_read<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, db, dp, dx, ds, dlu, dperm, ws, iws, n);
+  UNIT_SYNC();
+  UNIT_D2H(x, dx, n * N1 * sizeof(double));
+  UNIT_D2H(sing, ds, n * sizeof(double));
+  UNIT_D2H(lu, dlu, n * N1 * N1 * sizeof(double));
+  UNIT_D2H(perm, dperm, n * N1 * sizeof(int));
+  for (void* p : {(void*)dJ, (void*)dg, (void*)db, (void*)dp, (void*)dx, (void*)ds, (void*)dlu, (void*)dperm,
+                  (void*)ws, (void*)iws, (void*)dlist, (void*)dcount})
+    UNIT_DEV_FREE(p);
+  return 0;
+}
 }  // namespace
 }  // namespace pelelm
 
 using namespace pelelm;
+
+extern "C" int unit_lu(const char* mech, int coop, long n, const double* J, const double* gamma, const int* pend,
+                       const double* b, double* x, double* sing, double* lu, int* perm) {
+  const int m = mech_lookup(mech);
+  if (m == MECH_DRM19) return unit_lu_t<mech_drm19>(coop, n, J, gamma, pend, b, x, sing, lu, perm);
+  if (m == MECH_LIDRYER) return unit_lu_t<mech_LiDryer>(coop, n, J, gamma, pend, b, x, sing, lu, perm);
+  return -1;
+}
 
 extern "C" int unit_rhs(const char* mech, long n, const double* y, const double* t, const double* rE0,
                         const double* fH, const double* fy, double* f, double* T) {

@@ -1,8 +1,9 @@
 """Device components of the reactor on the GPU against the same code on the CPU.
 
 tests/gpu_unit/unit_kernels.hip wraps the reactor's device functions (right-hand side
-with HY2T, analytic Jacobian, LU factor/solve of I - gamma J, BDF coefficients and
-Nordsieck order changes) in bounded kernels; libunit_gpu.so is that file built for
+with HY2T, analytic Jacobian, LU factor/solve of I - gamma J -- one lane per cell and
+the cooperative 32-lanes-per-cell kernel -- BDF coefficients and Nordsieck order
+changes) in bounded kernels; libunit_gpu.so is that file built for
 gfx950, libunit_cpu.so the same file built for the host through the serial stand-in.
 The CPU twin is itself checked against the oracle in test_kernel_emulation.py.
 Jacobian/LU calls run in divergent lanes (lane % 3 != 0) while every lane carries a
@@ -138,6 +139,71 @@ def test_jac_lu_gpu_matches_cpu_in_divergent_lanes(mech, wl):
     # near-singular systems (|x| up to 1e7) amplify rounding: compare against each solution's scale
     xs = np.abs(xc[active]).max(axis=1, keepdims=True)
     assert np.max(np.abs(xg[active] - xc[active]) / xs) < 1e-10
+
+
+def run_lu(L, mech, coop, J, gamma, pend, b):
+    n, N1 = b.shape
+    x = np.zeros((n, N1))
+    sing = np.zeros(n)
+    lu = np.zeros((n, N1, N1))
+    perm = np.zeros((n, N1), dtype=np.int32)
+    rc = L.unit_lu(mech.encode(), coop, C.c_long(n), _p(J), _p(gamma), _p(pend), _p(b), _p(x), _p(sing), _p(lu),
+                   _p(perm))
+    assert rc == 0
+    return x, sing, lu, perm
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mech,wl", CASES)
+def test_cooperative_lu_matches_lane_lu(mech, wl):
+    """The cooperative LU (32 lanes per cell, two cells per wave) against the one-lane
+    factorisation on reactor Jacobians: same pivot order, same factors to rounding, a
+    valid permutation, the first zero pivot reported for singular matrices, and cells
+    with no set-up pending left untouched.  An odd cell count leaves the last wave half
+    empty."""
+    G, Cl = _load("libunit_gpu.so"), _load("libunit_cpu.so")
+    m, N1, y, t, rE0, fH, fy = _inputs(mech, wl, 645)
+    n = y.shape[0]
+    _, T = run_rhs(Cl, mech, y, t, rE0, fH, fy)
+    f, _ = run_rhs(Cl, mech, y, t, rE0, fH, fy)
+    rng = np.random.default_rng(12)
+    gamma = 10 ** rng.uniform(-10, -5, n)
+    b = rng.normal(size=(n, N1))
+    J, _, _, _ = run_jac_lu(Cl, mech, y, T, f[:, -1].copy(), gamma, b)
+    J = np.ascontiguousarray(J)
+    pend = (rng.random(n) < 0.8).astype(np.int32)
+    # singular matrices: column k of I - gamma J vanishes
+    sing_cells = np.arange(5, n, 97)
+    for i, c in enumerate(sing_cells):
+        k = (3 * i) % N1
+        J[c, :, k] = 0.0
+        J[c, k, k] = 1.0 / gamma[c]
+        pend[c] = 1
+    xc, sc, luc, pc = run_lu(G, mech, 1, J, gamma, pend, b)
+    xl, sl, lul, pl = run_lu(G, mech, 0, J, gamma, pend, b)
+    xh, sh, luh, ph = run_lu(Cl, mech, 0, J, gamma, pend, b)
+    _dump(f"lu_{mech}", xc=xc, xl=xl, sc=sc, sl=sl, pc=pc, pl=pl)
+    idle = pend == 0
+    assert np.all(sc[idle] == -3.0) and np.all(luc[idle] == 7.0) and np.all(pc[idle] == -1)
+    ok = (pend == 1) & (sl == 0)
+    is_sing = np.zeros(n, bool)
+    is_sing[sing_cells] = True
+    assert np.all(sl[is_sing] > 0) and np.all(sl[~is_sing & (pend == 1)] == 0)
+    np.testing.assert_array_equal(sc[pend == 1], sl[pend == 1])
+    np.testing.assert_array_equal(sl, sh)
+    for c in np.where(pend == 1)[0]:
+        assert sorted(pc[c]) == list(range(N1))
+    np.testing.assert_array_equal(pc[ok], pl[ok])
+    np.testing.assert_array_equal(pl[ok], ph[ok])
+    scale = np.abs(lul[ok]).max(axis=2, keepdims=True)
+    assert np.max(np.abs(luc[ok] - lul[ok]) / scale) < 1e-13
+    xs = np.abs(xl[ok]).max(axis=1, keepdims=True)
+    assert np.max(np.abs(xc[ok] - xl[ok]) / xs) < 1e-12
+    assert np.max(np.abs(xl[ok] - xh[ok]) / xs) < 1e-12
+    # and the solution solves the system
+    M = np.eye(N1)[None] - gamma[ok, None, None] * J[ok]
+    res = np.einsum("cij,cj->ci", M, xc[ok]) - b[ok]
+    assert np.max(np.abs(res) / (np.abs(M).max(axis=2) * xs + np.abs(b[ok]))) < 1e-12
 
 
 @pytest.mark.gpu
