@@ -35,6 +35,7 @@ struct Reactor {
   int schedule = SCHED_COMPACT;
   int lu_mode = LU_FP64;
   int ir_sweeps = 3;
+  int lu_lane = 0;
   bool on_fail_continue = true;
   bool profile = false;
   KernelProf prof{};
@@ -128,6 +129,7 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   a.mxstep = r->max_steps;
   a.lu_mode = r->lu_mode;
   a.ir_sweeps = r->ir_sweeps;
+  a.lu_lane = r->lu_lane;
   a.compact = (r->schedule == SCHED_COMPACT) ? 1 : 0;
   a.cell_status = nullptr;
   r->last_ncell = a.ncell;
@@ -261,8 +263,12 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
     else return fail(r, MI_EINVAL, "mi.schedule must be plain or queue");
   } else if (k == "mi.lu") {
     if (v == "fp64") r->lu_mode = LU_FP64;
-    else if (v == "fp64_lane") r->lu_mode = LU_FP64_LANE;
-    else return fail(r, MI_EINVAL, "mi.lu must be fp64 or fp64_lane");
+    else if (v == "fp32_ir") r->lu_mode = LU_FP32_IR;
+    else return fail(r, MI_EINVAL, "mi.lu must be fp64 or fp32_ir");
+  } else if (k == "mi.lu_form") {
+    if (v == "coop") r->lu_lane = 0;
+    else if (v == "lane") r->lu_lane = 1;
+    else return fail(r, MI_EINVAL, "mi.lu_form must be coop or lane");
   } else if (k == "mi.ir_sweeps") {
     if (!parse_long(value, &l) || l < 0 || l > 10) return fail(r, MI_EINVAL, "mi.ir_sweeps must be 0..10");
     r->ir_sweeps = (int)l;

@@ -46,6 +46,9 @@ constexpr int SMALL_NST = 10, MXNCF = 10, MXNEF = 7, MXNEF1 = 3, SMALL_NEF = 2, 
 constexpr int MAXCOR = 3, MSBP = 20, MSBJ = 51, HIN_ITERS = 4;
 constexpr int LMAX = 6;
 constexpr int HY2T_MAXIT = 50;
+// iterative refinement of FP32 factors: a sweep whose correction has WRMS norm (in
+// units of the error weights) at or below IR_TOL ends the refinement
+constexpr double IR_TOL = 1e-4;
 constexpr double HY2T_RTOL = 1e-10;
 
 enum { FIRST_CALL, PREV_CONV_FAIL, PREV_ERR_FAIL };
@@ -190,68 +193,87 @@ __device__ __noinline__ void jacobian(const double* __restrict__ y, const double
 // ---------------------------------------------------------------------------
 // dense LU of M = I - gamma J (partial pivoting, whole-row interchanges) and solve
 // ---------------------------------------------------------------------------
+// Factors of I - gamma J live in the LU slots: FP64 one per slot, or FP32 two per slot
+// (element e in half e & 1 of slot e >> 1, so a wave's loads of e and e + 1 share lines).
+template <int N1, class Real>
+struct LUAcc;
 template <int N1>
+struct LUAcc<N1, double> {
+  static __device__ __forceinline__ double& at(const WS& w, int e) { return w[Slots<N1>::LU + e]; }
+};
+template <int N1>
+struct LUAcc<N1, float> {
+  static __device__ __forceinline__ float& at(const WS& w, int e) {
+    return reinterpret_cast<float*>(&w[Slots<N1>::LU + (e >> 1)])[e & 1];
+  }
+};
+
+// one lane factors its cell's M = I - gamma J (partial pivoting, arithmetic in Real);
+// returns 0, or 1 + the first column without a nonzero pivot
+template <int N1, class Real>
 __device__ __noinline__ int lu_factor(const WS& ws, const IWS& iws, const double gamma) {
   using S = Slots<N1>;
+  using A = LUAcc<N1, Real>;
 #pragma unroll 1
   for (int i = 0; i < N1; ++i) {
     iws[S::PERM + i] = i;
 #pragma unroll
-    for (int j = 0; j < N1; ++j) ws[S::LU + i * N1 + j] = (i == j ? 1.0 : 0.0) - gamma * ws[S::JS + i * N1 + j];
+    for (int j = 0; j < N1; ++j) A::at(ws, i * N1 + j) = (Real)((i == j ? 1.0 : 0.0) - gamma * ws[S::JS + i * N1 + j]);
   }
   int sing = 0;
 #pragma unroll 1
   for (int k = 0; k < N1; ++k) {
     int p = k;
-    double amax = fabs(ws[S::LU + k * N1 + k]);
+    Real amax = fabs(A::at(ws, k * N1 + k));
 #pragma unroll 1
     for (int i = k + 1; i < N1; ++i) {
-      const double a = fabs(ws[S::LU + i * N1 + k]);
+      const Real a = fabs(A::at(ws, i * N1 + k));
       if (a > amax) { amax = a; p = i; }
     }
-    if (amax == 0.0) { if (!sing) sing = k + 1; continue; }
+    if (amax == (Real)0) { if (!sing) sing = k + 1; continue; }
     if (p != k) {
       const int t = iws[S::PERM + k];
       iws[S::PERM + k] = iws[S::PERM + p];
       iws[S::PERM + p] = t;
 #pragma unroll 1
       for (int j = 0; j < N1; ++j) {
-        const double t2 = ws[S::LU + k * N1 + j];
-        ws[S::LU + k * N1 + j] = ws[S::LU + p * N1 + j];
-        ws[S::LU + p * N1 + j] = t2;
+        const Real t2 = A::at(ws, k * N1 + j);
+        A::at(ws, k * N1 + j) = A::at(ws, p * N1 + j);
+        A::at(ws, p * N1 + j) = t2;
       }
     }
-    const double inv = 1.0 / ws[S::LU + k * N1 + k];
+    const Real inv = (Real)1 / A::at(ws, k * N1 + k);
 #pragma unroll 1
     for (int i = k + 1; i < N1; ++i) {
-      const double lik = ws[S::LU + i * N1 + k] * inv;
-      ws[S::LU + i * N1 + k] = lik;
-      if (lik != 0.0) {
+      const Real lik = A::at(ws, i * N1 + k) * inv;
+      A::at(ws, i * N1 + k) = lik;
+      if (lik != (Real)0) {
 #pragma unroll 1
-        for (int j = k + 1; j < N1; ++j) ws[S::LU + i * N1 + j] -= lik * ws[S::LU + k * N1 + j];
+        for (int j = k + 1; j < N1; ++j) A::at(ws, i * N1 + j) -= lik * A::at(ws, k * N1 + j);
       }
     }
   }
   return sing;
 }
 
-// solve (I - gamma J) x = b with b given in ws[TMP]; result in x (registers).
-// P M = L U with row i of the factors holding row PERM[i] of M.
-template <int N1>
+// solve M x = b with b given in ws[TMP]; result in x (registers), FP64 arithmetic on
+// factors stored in Real.  P M = L U with row i of the factors holding row PERM[i] of M.
+template <int N1, class Real = double>
 __device__ __forceinline__ void lu_solve(const WS& ws, const IWS& iws, double* __restrict__ x) {
   using S = Slots<N1>;
+  using A = LUAcc<N1, Real>;
 #pragma unroll
   for (int i = 0; i < N1; ++i) x[i] = ws[S::TMP + iws[S::PERM + i]];
 #pragma unroll
   for (int k = 0; k < N1; ++k)
 #pragma unroll
-    for (int i = k + 1; i < N1; ++i) x[i] -= ws[S::LU + i * N1 + k] * x[k];
+    for (int i = k + 1; i < N1; ++i) x[i] -= (double)A::at(ws, i * N1 + k) * x[k];
 #pragma unroll
   for (int k = N1 - 1; k >= 0; --k) {
     double s = x[k];
 #pragma unroll
-    for (int j = k + 1; j < N1; ++j) s -= ws[S::LU + k * N1 + j] * x[j];
-    x[k] = s / ws[S::LU + k * N1 + k];
+    for (int j = k + 1; j < N1; ++j) s -= (double)A::at(ws, k * N1 + j) * x[j];
+    x[k] = s / (double)A::at(ws, k * N1 + k);
   }
 }
 
@@ -288,6 +310,10 @@ struct Lane {
   // evaluation started an iteration-matrix set-up whose factorisation is to be consumed
   double rE0, fH;
   int setup_pending;
+  // linear algebra: lu_mode / ir_sweeps come from the call; lu64 = 1 once refinement of
+  // FP32 factors has failed for this cell (its later factorisations are FP64)
+  int lu_mode, ir_sweeps, lu64;
+  long nir, nlufb;
 
   __device__ __forceinline__ double& zn(int j, int i) const { return ws[S::ZN + j * N1 + i]; }
 
@@ -695,13 +721,57 @@ struct Lane {
     return true;
   }
 
+  // x = M^-1 b for b in ws[TMP] with FP32 factors of M = I - gammap J and up to
+  // ir_sweeps FP64-residual refinement sweeps (r = b - M x from the saved FP64 Jacobian,
+  // x += M^-1 r); false when the last correction is still above IR_TOL or not finite
+  __device__ bool solve_refined(double* x) {
+    double b[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) b[i] = ws[S::TMP + i];
+    lu_solve<N1, float>(ws, iws, x);
+    if (ir_sweeps == 0) return true;
+#pragma unroll 1
+    for (int sweep = 0; sweep < ir_sweeps; ++sweep) {
+      ++nir;
+#pragma unroll 1
+      for (int i = 0; i < N1; ++i) {
+        double r = b[i] - x[i];
+#pragma unroll
+        for (int j = 0; j < N1; ++j) r += gammap * ws[S::JS + i * N1 + j] * x[j];
+        ws[S::TMP + i] = r;
+      }
+      double d[N1];
+      lu_solve<N1, float>(ws, iws, d);
+#pragma unroll
+      for (int i = 0; i < N1; ++i) x[i] += d[i];
+      if (wrms_reg(d) <= IR_TOL) return true;
+    }
+    return false;   // also for a NaN norm
+  }
+
   // one Newton iteration with f = f(tn, zn0 + acor); returns true when tout is reached
   __device__ bool iterate(const double* f) {
     ++nni;
 #pragma unroll
     for (int i = 0; i < N1; ++i) ws[S::TMP + i] = gamma * f[i] - rl1 * zn(1, i) - ws[S::ACOR + i];
     double x[N1];
-    lu_solve<N1>(ws, iws, x);
+    if (lu_mode == LU_FP32_IR && !lu64) {
+      if (!solve_refined(x)) {
+        // the FP32 factors cannot deliver this solve: refactor in FP64 and restart the
+        // Newton iteration from the prediction (the Jacobian is kept unless stale)
+        ++nlufb;
+        lu64 = 1;
+        callSetup = true;
+        convfail = NO_FAILURES;
+        m = 0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) ws[S::ACOR + i] = 0.0;
+        a1 = 0.0; a2 = 0.0; teval = tn;
+        return false;
+      }
+    } else {
+      lu_solve<N1>(ws, iws, x);
+    }
     if (gamrat != 1.0) {
       const double sc = 2.0 / (1.0 + gamrat);
 #pragma unroll
@@ -762,6 +832,9 @@ struct Lane {
     status = ST_OK;
     phase = PH_INIT_F0;
     setup_pending = 0;
+    lu64 = 0;
+    nir = 0;
+    nlufb = 0;
     a1 = 0.0; a2 = 0.0; teval = 0.0;
   }
 };
@@ -788,6 +861,8 @@ __device__ __forceinline__ void load_lane(Lane<M>& st, const WS& w, const IWS& i
   st.ws = w;
   st.iws = iw;
   st.qmax = a.qmax;
+  st.lu_mode = a.lu_mode;
+  st.ir_sweeps = a.ir_sweeps;
   st.rtol = a.rtol;
   st.mxstep = a.mxstep;
   st.tout = a.dt;
@@ -860,6 +935,8 @@ __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
     st.ws = w;
     st.iws = cell_iws<M>(a.iws, c);
     st.qmax = a.qmax;
+    st.lu_mode = a.lu_mode;
+    st.ir_sweeps = a.ir_sweeps;
     st.rtol = a.rtol;
     st.mxstep = a.mxstep;
     st.tout = a.dt;
@@ -867,6 +944,8 @@ __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
       a.FC.p[cell_off(a.FC, ci, cj, ck)] = 0.0;
       st.phase = PH_NEED_CELL;
       st.status = ST_OK;
+      st.setup_pending = 0;
+      st.lu64 = 0;
       if (a.cell_status) a.cell_status[c] = 0;
     } else {
 #pragma unroll
@@ -953,24 +1032,39 @@ __global__ void __launch_bounds__(64) k_lu_lane(const ReactArgs a, const int* __
   Lane<M> st;
   load_lane(st, w, cell_iws<M>(a.iws, c), a);
   if (!st.setup_pending) return;
-  w[S::SING] = (double)lu_factor<M::N1>(w, st.iws, st.gamma);
+  const bool fp32 = (st.lu_mode == LU_FP32_IR) && !st.lu64;
+  w[S::SING] = (double)(fp32 ? lu_factor<M::N1, float>(w, st.iws, st.gamma)
+                             : lu_factor<M::N1, double>(w, st.iws, st.gamma));
 }
 
-// Cooperative LU of M = I - gamma J: 32 lanes per cell (row r of M in lane r), two
-// cells per wave.  Column k's pivot is the largest |m_rk| over the rows not yet
-// pivoted (a 32-lane max-reduction); its row is broadcast through LDS and every
+// an int field of the lane state, read straight from its workspace word
+template <class M>
+__device__ __forceinline__ int lane_int_field(const WS& w, size_t off) {
+  using S = Slots<M::N1>;
+  const double word = w[S::LANE + (int)(off / 8)];
+  int v;
+  __builtin_memcpy(&v, reinterpret_cast<const char*>(&word) + off % 8, sizeof(int));
+  return v;
+}
+
+// Cooperative LU of M = I - gamma J in Real arithmetic: 32 lanes per cell (row r of M
+// in lane r), two cells per wave.  Column k's pivot is the largest |m_rk| over the rows
+// not yet pivoted (a 32-lane max-reduction); its row is broadcast through LDS and every
 // other remaining row is eliminated in registers.  Row r ends as row `step` of the
 // factors, where `step` is the column at which it was chosen (PERM[step] = r), which
-// is partial pivoting with the interchanges kept as a permutation.
-template <class M>
+// is partial pivoting with the interchanges kept as a permutation.  A launch handles
+// the pending cells whose factor precision is Real (FP32 under LU_FP32_IR unless the
+// cell has fallen back to FP64).
+template <class M, class Real>
 __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __restrict__ list,
                                                 const unsigned int* __restrict__ count) {
   constexpr int N1 = M::N1;
   static_assert(N1 <= 32, "the cooperative LU holds one row per lane of a 32-lane half-wave");
   using S = Slots<N1>;
+  using A = LUAcc<N1, Real>;
+  constexpr bool IS32 = sizeof(Real) == 4;
   constexpr int GAMMA_WORD = (int)(offsetof(Lane<M>, gamma) / 8);
-  constexpr int PEND_WORD = (int)(offsetof(Lane<M>, setup_pending) / 8);
-  __shared__ double prow[2][N1];
+  __shared__ Real prow[2][N1];
   const int lane = threadIdx.x;
   const int half = lane >> 5, r = lane & 31;
   const long idx = (long)blockIdx.x * 2 + half;
@@ -981,50 +1075,48 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
   if (idx < n) {
     c = list[idx];
     const WS w = cell_ws<M>(a.ws, c);
-    int pend;
-    double pw = w[S::LANE + PEND_WORD];
-    __builtin_memcpy(&pend, reinterpret_cast<const char*>(&pw) + (offsetof(Lane<M>, setup_pending) % 8),
-                     sizeof(int));
-    work = (pend != 0);
+    const bool pend = lane_int_field<M>(w, offsetof(Lane<M>, setup_pending)) != 0;
+    const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && lane_int_field<M>(w, offsetof(Lane<M>, lu64)) == 0;
+    work = pend && (fp32_cell == IS32);
     gamma = w[S::LANE + GAMMA_WORD];
   }
   const bool row = work && r < N1;
-  double m[N1];
+  Real m[N1];
   if (row) {
     const WS w = cell_ws<M>(a.ws, c);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) m[j] = (r == j ? 1.0 : 0.0) - gamma * w[S::JS + r * N1 + j];
+    for (int j = 0; j < N1; ++j) m[j] = (Real)((r == j ? 1.0 : 0.0) - gamma * w[S::JS + r * N1 + j]);
   } else {
 #pragma unroll
-    for (int j = 0; j < N1; ++j) m[j] = 0.0;
+    for (int j = 0; j < N1; ++j) m[j] = (Real)0;
   }
   bool done = !row;
   int my_step = 0, sing = 0;
 #pragma unroll
   for (int k = 0; k < N1; ++k) {
-    double v = done ? -1.0 : fabs(m[k]);
-    if (v != v) v = HUGE_VAL;   // a NaN row still takes a definite place in the order
+    Real v = done ? (Real)-1 : fabs(m[k]);
+    if (v != v) v = (Real)HUGE_VAL;   // a NaN row still takes a definite place in the order
     int who = r;
 #pragma unroll
     for (int off = 16; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(v, off, 32);
+      const Real ov = __shfl_xor(v, off, 32);
       const int ow = __shfl_xor(who, off, 32);
       if (ov > v || (ov == v && ow < who)) {
         v = ov;
         who = ow;
       }
     }
-    if (v <= 0.0 && sing == 0) sing = k + 1;
+    if (v <= (Real)0 && sing == 0) sing = k + 1;
     if (row && r == who) {
 #pragma unroll
       for (int j = 0; j < N1; ++j) prow[half][j] = m[j];
     }
     __syncthreads();
-    if (!done && r != who && v > 0.0) {   // v: this column's pivot magnitude, equal in all 32 lanes
-      const double inv = 1.0 / prow[half][k];
-      const double l = m[k] * inv;
+    if (!done && r != who && v > (Real)0) {   // v: this column's pivot magnitude, equal in all 32 lanes
+      const Real inv = (Real)1 / prow[half][k];
+      const Real l = m[k] * inv;
       m[k] = l;
-      if (l != 0.0) {
+      if (l != (Real)0) {
 #pragma unroll
         for (int j = k + 1; j < N1; ++j) m[j] -= l * prow[half][j];
       }
@@ -1039,7 +1131,7 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
     const WS w = cell_ws<M>(a.ws, c);
     const IWS iw = cell_iws<M>(a.iws, c);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) w[S::LU + my_step * N1 + j] = m[j];
+    for (int j = 0; j < N1; ++j) A::at(w, my_step * N1 + j) = m[j];
     iw[S::PERM + my_step] = r;
     if (r == 0) w[S::SING] = (double)sing;
   }
@@ -1056,7 +1148,7 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
   const long n = (long)*count;
   bool keep = false, want_setup = false;
   unsigned long long t_cells = 0, t_nst = 0, t_nfe = 0, t_nje = 0, t_nsetups = 0, t_nni = 0, t_ncfn = 0,
-                     t_netf = 0, t_fail = 0, t_maxst = 0;
+                     t_netf = 0, t_fail = 0, t_maxst = 0, t_ir = 0, t_lufb = 0;
   long c = -1;
   if (idx < n) {
     c = list[idx];
@@ -1120,6 +1212,7 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
         t_cells = 1;
         t_nst = st.nst; t_nfe = st.nfe; t_nje = st.nje; t_nsetups = st.nsetups; t_nni = st.nni;
         t_ncfn = st.ncfn; t_netf = st.netf; t_fail = ok ? 0 : 1; t_maxst = st.nst;
+        t_ir = st.nir; t_lufb = st.nlufb;
       } else {
         keep = true;
         want_setup = st.needs_setup();
@@ -1146,6 +1239,8 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
     wave_add(&a.counters[CNT_NETF], t_netf);
     wave_add(&a.counters[CNT_NFAIL], t_fail);
     wave_max(&a.counters[CNT_MAXSTEPS], t_maxst);
+    wave_add(&a.counters[CNT_IR], t_ir);
+    wave_add(&a.counters[CNT_LUFALLBACK], t_lufb);
   }
 }
 
@@ -1256,10 +1351,18 @@ __global__ void null_kernel(const ReactArgs a, int nspec) {
 
 template <class M>
 hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* count, long n, hipStream_t stream) {
-  if (a.lu_mode == LU_FP64_LANE)
+  if (a.lu_lane) {
     hipLaunchKernelGGL(k_lu_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, list, count);
-  else
-    hipLaunchKernelGGL(k_lu_coop<M>, dim3((unsigned)((n + 1) / 2)), dim3(64), 0, stream, a, list, count);
+    return hipGetLastError();
+  }
+  const dim3 grid((unsigned)((n + 1) / 2));
+  if (a.lu_mode == LU_FP32_IR) {
+    hipLaunchKernelGGL((k_lu_coop<M, float>), grid, dim3(64), 0, stream, a, list, count);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  // FP64 factorisations: every pending cell, or under LU_FP32_IR the cells that fell back
+  hipLaunchKernelGGL((k_lu_coop<M, double>), grid, dim3(64), 0, stream, a, list, count);
   return hipGetLastError();
 }
 

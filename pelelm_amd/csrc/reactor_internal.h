@@ -24,9 +24,9 @@ struct IntView {
 };
 
 enum Schedule : int { SCHED_PLAIN = 0, SCHED_COMPACT = 1 };
-// LU_FP64: cooperative FP64 factorisation (32 lanes per cell); LU_FP64_LANE: one lane
-// per cell (the form that runs when the kernels execute serially on a CPU)
-enum LuMode : int { LU_FP64 = 0, LU_FP32_IR = 1, LU_FP64_LANE = 2 };
+// LU_FP64: FP64 factors of I - gamma J; LU_FP32_IR: FP32 factors with FP64-residual
+// iterative refinement of every solve (a cell whose refinement fails falls back to FP64)
+enum LuMode : int { LU_FP64 = 0, LU_FP32_IR = 1 };
 
 // Per-call options shared by every schedule.
 struct ReactArgs {
@@ -41,6 +41,8 @@ struct ReactArgs {
   long mxstep;
   int lu_mode;
   int ir_sweeps;
+  int lu_lane;                           // 1: one lane factors a cell (the form that runs when
+                                         // the kernels execute serially on a CPU); 0: 32 lanes
   // per-cell workspace, tiled by 64 cells: element (slot s) of cell c at
   // ws[((c / 64) * nslot + s) * 64 + c % 64]
   double* ws;

@@ -7,6 +7,13 @@ thread_local Idx3 threadIdx, blockIdx, blockDim;
 
 using namespace pelelm;
 
+static int g_lu_mode = LU_FP64, g_ir_sweeps = 3;
+// factor precision of the following emu_react calls (LU_FP64 = 0, LU_FP32_IR = 1)
+extern "C" void emu_set_lu(int mode, int ir_sweeps) {
+  g_lu_mode = mode;
+  g_ir_sweeps = ir_sweeps;
+}
+
 extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, double* rE, double* FC, double dt,
                          double rtol, double atol, int max_order, int compact, unsigned long long* counters,
                          long* trips) {
@@ -34,7 +41,9 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   }
   a.counts = counts;
   a.compact = compact;
-  a.lu_mode = LU_FP64_LANE;   // lanes run one at a time here, so they cannot cooperate
+  a.lu_lane = 1;   // lanes run one at a time here, so they cannot cooperate
+  a.lu_mode = g_lu_mode;
+  a.ir_sweeps = g_ir_sweeps;
   a.counters = counters;
   const hipError_t e = run_react(m, a, nullptr, 3 * 20000 + 100, trips, nullptr);
   delete[] ws;

@@ -74,7 +74,7 @@ __global__ void k_jac_lu(const double* y, const double* T, const double* fT, con
     double yy[N1];
     for (int i = 0; i < N1; ++i) yy[i] = y[c * N1 + i];
     jacobian<M>(yy, T[c], fT[c], w);
-    sg = lu_factor<N1>(w, iw, gamma[c]);
+    sg = lu_factor<N1, double>(w, iw, gamma[c]);
     for (int i = 0; i < N1; ++i) w[S::TMP + i] = b[c * N1 + i];
     double xx[N1];
     lu_solve<N1>(w, iw, xx);
@@ -301,7 +301,7 @@ __global__ void k_lu_prep(const double* J, const double* gamma, const int* pend,
   w[S::SING] = -3.0;
 }
 
-template <class M>
+template <class M, class Real>
 __global__ void k_lu_read(const double* b, const int* pend, double* x, double* sing, double* lu, int* perm, double* ws,
                           int* iws, long ncell) {
   constexpr int N1 = M::N1;
@@ -311,18 +311,20 @@ __global__ void k_lu_read(const double* b, const int* pend, double* x, double* s
   const WS w = cell_ws<M>(ws, c);
   const IWS iw = cell_iws<M>(iws, c);
   sing[c] = w[S::SING];
-  for (int e = 0; e < N1 * N1; ++e) lu[c * N1 * N1 + e] = w[S::LU + e];
+  const bool untouched = (w[S::LU] == 7.0);   // idle cells keep the marker in every slot
+  for (int e = 0; e < N1 * N1; ++e)
+    lu[c * N1 * N1 + e] = untouched ? w[S::LU + e] : (double)LUAcc<N1, Real>::at(w, e);
   for (int i = 0; i < N1; ++i) perm[c * N1 + i] = iw[S::PERM + i];
   if (pend[c] && w[S::SING] == 0.0) {
     for (int i = 0; i < N1; ++i) w[S::TMP + i] = b[c * N1 + i];
     double xx[N1];
-    lu_solve<N1>(w, iw, xx);
+    lu_solve<N1, Real>(w, iw, xx);
     for (int i = 0; i < N1; ++i) x[c * N1 + i] = xx[i];
   }
 }
 
 template <class M>
-int unit_lu_t(int coop, long n, const double* J, const double* gamma, const int* pend, const double* b, double* x,
+int unit_lu_t(int coop, int fp32, long n, const double* J, const double* gamma, const int* pend, const double* b, double* x,
               double* sing, double* lu, int* perm) {
   constexpr int N1 = M::N1;
 #ifdef UNIT_HOST
@@ -344,10 +346,16 @@ int unit_lu_t(int coop, long n, const double* J, const double* gamma, const int*
   __builtin_memset(&a, 0, sizeof(a));
   a.ws = ws;
   a.iws = iws;
-  a.lu_mode = coop ? LU_FP64 : LU_FP64_LANE;
+  a.lu_mode = fp32 ? LU_FP32_IR : LU_FP64;
+  a.lu_lane = coop ? 0 : 1;
   hipLaunchKernelGGL(k_lu_prep<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, dJ, dg, dp, ws, iws, n);
   (void)launch_lu<M>(a, dlist, dcount, n, nullptr);
-  hipLaunchKernelGGL(k_lu_read<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, db, dp, dx, ds, dlu, dperm, ws, iws, n);
+  if (fp32)
+    hipLaunchKernelGGL((k_lu_read<M, float>), dim3((unsigned)nw), dim3(64), 0, nullptr, db, dp, dx, ds, dlu, dperm, ws,
+                       iws, n);
+  else
+    hipLaunchKernelGGL((k_lu_read<M, double>), dim3((unsigned)nw), dim3(64), 0, nullptr, db, dp, dx, ds, dlu, dperm, ws,
+                       iws, n);
   UNIT_SYNC();
   UNIT_D2H(x, dx, n * N1 * sizeof(double));
   UNIT_D2H(sing, ds, n * sizeof(double));
@@ -363,11 +371,11 @@ int unit_lu_t(int coop, long n, const double* J, const double* gamma, const int*
 
 using namespace pelelm;
 
-extern "C" int unit_lu(const char* mech, int coop, long n, const double* J, const double* gamma, const int* pend,
-                       const double* b, double* x, double* sing, double* lu, int* perm) {
+extern "C" int unit_lu(const char* mech, int coop, int fp32, long n, const double* J, const double* gamma,
+                       const int* pend, const double* b, double* x, double* sing, double* lu, int* perm) {
   const int m = mech_lookup(mech);
-  if (m == MECH_DRM19) return unit_lu_t<mech_drm19>(coop, n, J, gamma, pend, b, x, sing, lu, perm);
-  if (m == MECH_LIDRYER) return unit_lu_t<mech_LiDryer>(coop, n, J, gamma, pend, b, x, sing, lu, perm);
+  if (m == MECH_DRM19) return unit_lu_t<mech_drm19>(coop, fp32, n, J, gamma, pend, b, x, sing, lu, perm);
+  if (m == MECH_LIDRYER) return unit_lu_t<mech_LiDryer>(coop, fp32, n, J, gamma, pend, b, x, sing, lu, perm);
   return -1;
 }
 

@@ -80,3 +80,112 @@ def test_react_matches_oracle(wl, mech, n, schedule):
     assert np.mean(gFC == ref["FC"]) > 0.9
     # mass conservation is exact up to rounding
     np.testing.assert_allclose(gY.sum(0), d["rY"].sum(0), rtol=1e-12)
+
+
+@pytest.mark.parametrize("wl,mech", [("drm19_64", "drm19"), ("lidryer_16", "LiDryer")])
+def test_fp32_ir_matches_fp64_and_tight_oracle(wl, mech):
+    """Mixed precision (FP32 factors, up to 3 FP64-residual refinement sweeps per solve)
+    against the FP64 oracle at the same rtol and against the tight oracle (rtol 1e-12):
+    it must add no error beyond what the rtol-1e-6 integration itself makes."""
+    n = 2048
+    d = cells.generate_workload(wl, ncell=n)
+    m = O.Mech(mech)
+    ref = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    tight = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-12, atol=1e-20, max_steps=200000)
+    r = _mk(mech, ode__rtol=1e-6, ode__atol=1e-10, mi__lu="fp32_ir", mi__ir_sweeps=3)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    assert st["nfail"] == 0 and st["nlu_refine"] > 0
+    gY, gT = rY.cpu().numpy(), T.cpu().numpy()
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    assert (np.abs(gY - ref["rY"]) / tol).max() < 10.0
+    assert np.abs(gT - ref["T"]).max() < 1e-2
+    e_ir = np.abs(gT - tight["T"]).max()
+    e_64 = np.abs(ref["T"] - tight["T"]).max()
+    print(f"{wl}: max|dT| vs tight oracle: fp32_ir {e_ir:.3g} K, fp64 {e_64:.3g} K; "
+          f"refinement sweeps {st['nlu_refine']}, FP64 fallbacks {st['nlu_fallback']}")
+    assert e_ir <= 1.01 * e_64 + 1e-6
+
+
+def test_forcing_matches_oracle():
+    n = 1024
+    d = cells.generate_workload("drm19_64", ncell=n)
+    m = O.Mech("drm19")
+    rng = np.random.default_rng(5)
+    fY = rng.normal(size=d["rY"].shape) * 1e-3 * d["rY"]
+    fH = rng.normal(size=n) * 1e6
+    ref = m.react(d["rY"], d["T"], d["rE"], d["dt"], rYsrc=fY, rEsrc=fH, rtol=1e-6, atol=1e-10)
+    r = _mk("drm19", ode__rtol=1e-6, ode__atol=1e-10)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, _dev(fY), T, rE, _dev(fH), FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    assert r.stats()["nfail"] == 0
+    r.close()
+    gY, gT, gE = rY.cpu().numpy(), T.cpu().numpy(), rE.cpu().numpy()
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    assert (np.abs(gY - ref["rY"]) / tol).max() < 10.0
+    assert np.abs(gT - ref["T"]).max() < 1e-2
+    np.testing.assert_allclose(gE, d["rE"] + d["dt"] * fH, rtol=1e-15)
+    np.testing.assert_allclose(gY.sum(0), d["rY"].sum(0) + d["dt"] * fY.sum(0), rtol=1e-12)
+
+
+def test_mask_skips_covered_cells():
+    n = 1024
+    d = cells.generate_workload("lidryer_16", ncell=n)
+    mask = np.zeros(n, dtype=np.int32)
+    mask[::3] = -1
+    ref = O.Mech("LiDryer").react(d["rY"], d["T"], d["rE"], d["dt"], mask=mask)
+    r = _mk("LiDryer", ode__rtol=1e-6, ode__atol=1e-10)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.full((n,), 123.0, dtype=torch.float64, device=DEV)
+    r.react(rY, None, T, rE, None, FC, _dev(mask, torch.int32), d["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    assert st["ncells"] == n - len(mask[::3])
+    gY, gT, gFC = rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy()
+    np.testing.assert_array_equal(gY[:, ::3], d["rY"][:, ::3])
+    np.testing.assert_array_equal(gT[::3], d["T"][::3])
+    act = mask >= 0
+    tol = 1e-6 * np.abs(ref["rY"][:, act]) + 1e-10
+    assert (np.abs(gY[:, act] - ref["rY"][:, act]) / tol).max() < 10.0
+    assert np.all(gFC[act] > 0)
+
+
+def test_react_box_on_array4_views_with_ghost_cells():
+    """A 12^3 valid box inside a 16^3 fab (2 ghost cells each side), components in planes
+    like PeleLM's STemp (rhoY, rhoH, T, Source/PeleLM.cpp:5728-5730): only valid cells
+    change, and they match the flat call."""
+    nv, ng = 12, 2
+    nf = nv + 2 * ng
+    d = cells.generate_workload("drm19_64", ncell=nf ** 3)
+    N = d["rY"].shape[0]
+    fab = np.concatenate([d["rY"], d["rE"][None], d["T"][None]], axis=0).reshape(N + 2, nf, nf, nf)
+    S = _dev(fab)
+    FCf = torch.zeros((1, nf, nf, nf), dtype=torch.float64, device=DEV)
+    lo_fab = (-ng, -ng, -ng)
+    r = _mk("drm19", ode__rtol=1e-6, ode__atol=1e-10)
+    v = lambda t, nc: reactor.array4(t, lo_fab, None, nc)  # noqa: E731
+    rYv, rEv, Tv = v(S[:N], N), v(S[N:N + 1], 1), v(S[N + 1:N + 2], 1)
+    z = reactor.null_real4()
+    r.react_box((0, 0, 0), (nv - 1, nv - 1, nv - 1), rYv, z, Tv, rEv, z, v(FCf, 1), reactor.int4(None, lo_fab),
+                d["dt"], 0.0)
+    torch.cuda.synchronize()
+    assert r.stats()["ncells"] == nv ** 3
+    r.close()
+    out = S.cpu().numpy()
+    inner = (slice(None), slice(ng, ng + nv), slice(ng, ng + nv), slice(ng, ng + nv))
+    ghost = np.ones((nf, nf, nf), bool)
+    ghost[ng:ng + nv, ng:ng + nv, ng:ng + nv] = False
+    np.testing.assert_array_equal(out[:, ghost], fab[:, ghost])
+    flat_in = fab[inner].reshape(N + 2, -1)
+    ref = O.Mech("drm19").react(flat_in[:N], flat_in[N + 1], flat_in[N], d["dt"], rtol=1e-6, atol=1e-10)
+    got = out[inner].reshape(N + 2, -1)
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    assert (np.abs(got[:N] - ref["rY"]) / tol).max() < 10.0
+    assert np.abs(got[N + 1] - ref["T"]).max() < 1e-2

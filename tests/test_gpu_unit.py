@@ -141,16 +141,59 @@ def test_jac_lu_gpu_matches_cpu_in_divergent_lanes(mech, wl):
     assert np.max(np.abs(xg[active] - xc[active]) / xs) < 1e-10
 
 
-def run_lu(L, mech, coop, J, gamma, pend, b):
+def run_lu(L, mech, coop, J, gamma, pend, b, fp32=0):
     n, N1 = b.shape
     x = np.zeros((n, N1))
     sing = np.zeros(n)
     lu = np.zeros((n, N1, N1))
     perm = np.zeros((n, N1), dtype=np.int32)
-    rc = L.unit_lu(mech.encode(), coop, C.c_long(n), _p(J), _p(gamma), _p(pend), _p(b), _p(x), _p(sing), _p(lu),
-                   _p(perm))
+    rc = L.unit_lu(mech.encode(), coop, fp32, C.c_long(n), _p(J), _p(gamma), _p(pend), _p(b), _p(x), _p(sing),
+                   _p(lu), _p(perm))
     assert rc == 0
     return x, sing, lu, perm
+
+
+def _lu_case(mech, wl):
+    Cl = _load("libunit_cpu.so")
+    m, N1, y, t, rE0, fH, fy = _inputs(mech, wl, 645)
+    n = y.shape[0]
+    _, T = run_rhs(Cl, mech, y, t, rE0, fH, fy)
+    f, _ = run_rhs(Cl, mech, y, t, rE0, fH, fy)
+    rng = np.random.default_rng(12)
+    gamma = 10 ** rng.uniform(-10, -5, n)
+    b = rng.normal(size=(n, N1))
+    J, _, _, _ = run_jac_lu(Cl, mech, y, T, f[:, -1].copy(), gamma, b)
+    J = np.ascontiguousarray(J)
+    pend = (rng.random(n) < 0.8).astype(np.int32)
+    return N1, n, J, gamma, pend, b
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mech,wl", CASES)
+def test_cooperative_fp32_lu_matches_lane_fp32_lu(mech, wl):
+    """FP32 factors (two per workspace slot) from the cooperative kernel against the
+    one-lane FP32 factorisation, on the GPU and on the CPU twin; the FP32 solve's error
+    against numpy is what the refinement sweeps then remove."""
+    G, Cl = _load("libunit_gpu.so"), _load("libunit_cpu.so")
+    N1, n, J, gamma, pend, b = _lu_case(mech, wl)
+    xc, sc, luc, pc = run_lu(G, mech, 1, J, gamma, pend, b, fp32=1)
+    xl, sl, lul, pl = run_lu(G, mech, 0, J, gamma, pend, b, fp32=1)
+    xh, sh, luh, ph = run_lu(Cl, mech, 0, J, gamma, pend, b, fp32=1)
+    idle = pend == 0
+    assert np.all(sc[idle] == -3.0) and np.all(luc[idle] == 7.0) and np.all(pc[idle] == -1)
+    ok = pend == 1
+    assert np.all(sc[ok] == 0) and np.all(sl[ok] == 0) and np.all(sh[ok] == 0)
+    # same pivots wherever the FP32 column maxima are not tied
+    assert np.mean(np.all(pc[ok] == pl[ok], axis=1)) > 0.99
+    np.testing.assert_array_equal(pl[ok], ph[ok])
+    same = np.all(pc == pl, axis=1) & ok
+    scale = np.abs(lul[same]).max(axis=2, keepdims=True)
+    assert np.max(np.abs(luc[same] - lul[same]) / scale) < 1e-5
+    np.testing.assert_allclose(luh[ok], lul[ok], rtol=0, atol=0)
+    M = np.eye(N1)[None] - gamma[ok, None, None] * J[ok]
+    xe = np.linalg.solve(M, b[ok][..., None])[..., 0]
+    xs = np.abs(xe).max(axis=1, keepdims=True)
+    assert np.max(np.abs(xc[ok] - xe) / xs) < 1e-3
 
 
 @pytest.mark.gpu
