@@ -25,6 +25,16 @@ def test_library_exports_every_declared_symbol():
     assert sorted(reactor.EXPORTS) == syms
 
 
+def test_library_exports_every_network_symbol():
+    from pelelm_amd import predictor
+    text = open(os.path.join(ROOT, "include", "pelelm_mlp.h")).read()
+    syms = sorted(set(re.findall(r"\b(mi_mlp_\w+)\s*\(", text)))
+    lib = reactor.lib()
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(predictor.MLP_EXPORTS) == syms
+
+
 def test_create_rejects_unknown_integrator():
     with pytest.raises(reactor.ReactorError):
         reactor.ReactorBase.create("ReactorRK64")
