@@ -30,6 +30,8 @@ extern "C" {
 #define MI_ECELLFAIL -4   /* at least one cell failed to integrate          */
 #define MI_ENOMEM -5
 
+#define MI_NKERNEL_KINDS 5
+
 /* amrex::Array4<Real>-compatible view: element (i,j,k,n) lives at
    p[(i-begin[0]) + (j-begin[1])*jstride + (k-begin[2])*kstride + n*nstride]. */
 typedef struct {
@@ -62,10 +64,11 @@ typedef struct {
   long ntrips;          /* right-hand-side trips the call took                  */
   long nhy2t_iters;     /* Newton iterations of the T-from-enthalpy solves      */
   /* with mi.profile = 1: device time (ms), launches and cells processed per kernel
-     kind [0] right-hand side, [1] Jacobian set-up, [2] Newton/step logic, [3] LU */
-  double kernel_ms[4];
-  long kernel_launches[4];
-  long kernel_cells[4];
+     kind [0] right-hand side, [1] Jacobian set-up, [2] Newton/step logic, [3] LU,
+     [4] learned predictor */
+  double kernel_ms[MI_NKERNEL_KINDS];
+  long kernel_launches[MI_NKERNEL_KINDS];
+  long kernel_cells[MI_NKERNEL_KINDS];
 } mi_chem_stats;
 
 /* ReactorBase::create(name) -- Source/PeleLM.cpp:706.

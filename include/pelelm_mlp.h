@@ -47,6 +47,14 @@ int mi_mlp_grad(void* mlp, const float* X, const float* Yt, long B, float* grad,
 /* One Adam step on that loss with learning rate lr. */
 int mi_mlp_train_step(void* mlp, const float* X, const float* Yt, long B, float lr, float* loss, void* hip_stream);
 
+/* Attach a network to a reactor; a null network detaches it.  Each BDF Newton
+   attempt then starts from the network's prediction of the state over the attempted
+   step, with the Nordsieck prediction kept as the error-test reference and as the
+   fallback when the Newton iteration fails.  Shape: (nspec + 3) inputs (T, log p,
+   log10 Y_k, log h) and (nspec + 1) outputs (dT, dY_k), normalised as set by
+   mi_mlp_set_normalisation.  Needs mi.schedule = queue.  The network is not owned. */
+int mi_chem_attach_predictor(void* reactor, void* mlp);
+
 const char* mi_mlp_last_error(void* mlp);
 int mi_mlp_close(void* mlp);
 

@@ -12,6 +12,8 @@
 #include <vector>
 
 #include "../../include/pelelm_chem.h"
+#include "../../include/pelelm_mlp.h"
+#include "mlp_internal.h"
 #include "reactor_internal.h"
 
 using namespace pelelm;
@@ -36,6 +38,7 @@ struct Reactor {
   int lu_mode = LU_FP64;
   int ir_sweeps = 3;
   int lu_lane = 0;
+  void* predictor = nullptr;    // attached network (mi_chem_attach_predictor), not owned
   bool on_fail_continue = true;
   bool profile = false;
   KernelProf prof{};
@@ -44,7 +47,7 @@ struct Reactor {
   int ncu = 0;
   double* ws = nullptr;
   int* iws = nullptr;
-  int* lists = nullptr;         // 4 lists of ws_cells ints
+  int* lists = nullptr;         // 5 lists of ws_cells ints (active x2, set-up x2, predictor)
   long ws_cells = 0;            // cells the workspace and lists hold
   unsigned int* counts = nullptr;
   unsigned long long* counters = nullptr;
@@ -93,7 +96,7 @@ int ensure_device_state(Reactor* r, long ncell) {
     if (hipMalloc(&r->ws, nd * sizeof(double)) != hipSuccess)
       return fail(r, MI_ENOMEM, "workspace allocation failed (" + std::to_string(nd * 8) + " bytes)");
     if (hipMalloc(&r->iws, ni * sizeof(int)) != hipSuccess) return fail(r, MI_ENOMEM, "workspace allocation failed");
-    if (hipMalloc(&r->lists, 4 * (size_t)nw * 64 * sizeof(int)) != hipSuccess)
+    if (hipMalloc(&r->lists, 5 * (size_t)nw * 64 * sizeof(int)) != hipSuccess)
       return fail(r, MI_ENOMEM, "list allocation failed");
     r->ws_cells = nw * 64;
   }
@@ -145,6 +148,9 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
     a.list[i] = r->lists + (size_t)i * r->ws_cells;
     a.slist[i] = r->lists + (size_t)(2 + i) * r->ws_cells;
   }
+  a.plist = r->lists + (size_t)4 * r->ws_cells;
+  a.use_pred = (r->predictor && !r->null_reactor) ? 1 : 0;
+  if (a.use_pred && !a.compact) return fail(r, MI_EINVAL, "the learned predictor needs mi.schedule = queue");
   a.counts = r->counts;
   a.counters = r->counters;
   HIPCHK(r, launch_counters_reset(r->counters, stream));
@@ -155,7 +161,8 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
     // every step takes at least one trip; 3 per step bounds retries generously
     const long max_trips = 3 * r->max_steps + 100;
     r->prof = KernelProf{};
-    HIPCHK(r, run_react(r->mech, a, stream, max_trips, &r->last_trips, r->profile ? &r->prof : nullptr));
+    HIPCHK(r, run_react(r->mech, a, stream, max_trips, &r->last_trips, r->profile ? &r->prof : nullptr,
+                        mlp_device(r->predictor)));
   }
   HIPCHK(r, hipEventRecord(r->ev1, stream));
   r->last_stream = stream;
@@ -377,16 +384,34 @@ int mi_chem_get_stats(void* reactor, mi_chem_stats* out) {
   out->nlu_refine = (long)c[CNT_IR];
   out->nlu_fallback = (long)c[CNT_LUFALLBACK];
   out->nhy2t_iters = (long)c[CNT_HY2T];
+  out->npred_used = (long)c[CNT_PRED];
+  out->npred_fallback = (long)c[CNT_PREDFB];
   float ms = 0.f;
   HIPCHK(r, hipEventElapsedTime(&ms, r->ev0, r->ev1));
   out->ms_kernel = ms;
   out->ntrips = r->last_trips;
-  for (int kk = 0; kk < 4 && kk < NKK; ++kk) {
+  for (int kk = 0; kk < MI_NKERNEL_KINDS && kk < NKK; ++kk) {
     out->kernel_ms[kk] = r->prof.ms[kk];
     out->kernel_launches[kk] = r->prof.launches[kk];
     out->kernel_cells[kk] = r->prof.cells[kk];
   }
   if (out->nfail > 0 && !r->on_fail_continue) return fail(r, MI_ECELLFAIL, "cells failed to integrate");
+  return MI_OK;
+}
+
+int mi_chem_attach_predictor(void* reactor, void* mlp) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (!mlp) {
+    r->predictor = nullptr;
+    return MI_OK;
+  }
+  const MlpDev* d = mlp_device(mlp);
+  const int N = mech_nspec(r->mech);
+  if (d->n_in != N + 3 || d->n_out != N + 1)
+    return fail(r, MI_EINVAL, "network shape must be (nspec + 3) inputs and (nspec + 1) outputs");
+  if (r->schedule != SCHED_COMPACT) return fail(r, MI_EINVAL, "the learned predictor needs mi.schedule = queue");
+  r->predictor = mlp;
   return MI_OK;
 }
 

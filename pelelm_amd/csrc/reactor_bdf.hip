@@ -26,6 +26,7 @@
 #include <cstddef>
 
 #include "reactor_internal.h"
+#include "mlp_internal.h"
 
 #ifndef PELE_WAVES_PER_EU
 #define PELE_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
@@ -314,6 +315,10 @@ struct Lane {
   // FP32 factors has failed for this cell (its later factorisations are FP64)
   int lu_mode, ir_sweeps, lu64;
   long nir, nlufb;
+  // learned predictor: use_pred comes from the call; pred_state 0 none, 1 requested for
+  // this attempt (y_n saved in TMP), 2 applied (acor holds prediction - zn0), 3 declined
+  int use_pred, pred_state;
+  long npred, npredfb;
 
   __device__ __forceinline__ double& zn(int j, int i) const { return ws[S::ZN + j * N1 + i]; }
 
@@ -451,6 +456,14 @@ struct Lane {
 
   // start one attempt of the current step: predict, coefficients, Newton set-up
   __device__ void attempt() {
+    if (use_pred) {
+      // the state at tn (before the Nordsieck prediction) is the network's input
+#pragma unroll
+      for (int i = 0; i < N1; ++i) ws[S::TMP + i] = zn(0, i);
+      pred_state = 1;
+    } else {
+      pred_state = 0;
+    }
     predict();
     set_coeffs();
     convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
@@ -555,6 +568,18 @@ struct Lane {
     attempt();
   }
   __device__ void nls_fail() {
+    if (pred_state == 2) {
+      // the learned initial iterate did not converge: retry the same step from the
+      // Nordsieck prediction (the matrix set up in this attempt stays valid)
+      ++npredfb;
+      pred_state = 3;
+      callSetup = false;
+      m = 0;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) ws[S::ACOR + i] = 0.0;
+      a1 = 0.0; a2 = 0.0; teval = tn;
+      return;
+    }
     if (!jcur) {
       callSetup = true;
       convfail = FAIL_BAD_J;
@@ -783,7 +808,8 @@ struct Lane {
     if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
     const double dcon = del * fmin(1.0, crate) / tq[4];
     if (dcon <= 1.0) {
-      acnrm = (m == 0) ? del : wrms_ws(S::ACOR);
+      // with a learned initial iterate acor is not the first correction alone
+      acnrm = (m == 0 && pred_state != 2) ? del : wrms_ws(S::ACOR);
       jcur = false;
       return after_converged();
     }
@@ -835,6 +861,9 @@ struct Lane {
     lu64 = 0;
     nir = 0;
     nlufb = 0;
+    pred_state = 0;
+    npred = 0;
+    npredfb = 0;
     a1 = 0.0; a2 = 0.0; teval = 0.0;
   }
 };
@@ -863,6 +892,7 @@ __device__ __forceinline__ void load_lane(Lane<M>& st, const WS& w, const IWS& i
   st.qmax = a.qmax;
   st.lu_mode = a.lu_mode;
   st.ir_sweeps = a.ir_sweeps;
+  st.use_pred = a.use_pred;
   st.rtol = a.rtol;
   st.mxstep = a.mxstep;
   st.tout = a.dt;
@@ -937,6 +967,7 @@ __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
     st.qmax = a.qmax;
     st.lu_mode = a.lu_mode;
     st.ir_sweeps = a.ir_sweeps;
+    st.use_pred = a.use_pred;
     st.rtol = a.rtol;
     st.mxstep = a.mxstep;
     st.tout = a.dt;
@@ -946,6 +977,7 @@ __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
       st.status = ST_OK;
       st.setup_pending = 0;
       st.lu64 = 0;
+      st.pred_state = 0;
       if (a.cell_status) a.cell_status[c] = 0;
     } else {
 #pragma unroll
@@ -1017,6 +1049,104 @@ __global__ void __launch_bounds__(64) k_setup(const ReactArgs a, const int* __re
   }
   st.setup(y, f, w[S::TS]);
   store_lane(st, w);
+}
+
+// Learned initial iterate for the cells whose attempt requested one (pred_state 1):
+// features (T, log p, log10 Y_k, log h) of the state y_n saved in TMP, normalised;
+// the network (three GELU layers of 128 on the matrix cores) gives (dT, dY_k) over h;
+// acor = (T + dT, rho (Y + dY)) - zn0, so the first evaluation is at the learned guess
+// while the error test keeps measuring against the Nordsieck prediction zn0.
+// 64 cells per workgroup of four waves.
+template <class M, int KIN, int NOUT>
+__global__ void __launch_bounds__(MLP_THREADS) k_predict(const ReactArgs a, const MlpDev w,
+                                                         const int* __restrict__ list,
+                                                         const unsigned int* __restrict__ count) {
+  constexpr int N = M::NS, N1 = M::N1;
+  static_assert(N + 3 <= KIN && N1 <= NOUT, "network shape does not fit the mechanism");
+  using S = Slots<N1>;
+  constexpr int H_WORD = (int)(offsetof(Lane<M>, h) / 8);
+  __shared__ unsigned short h0[MLP_ROWS * MLP_LDW], h1[MLP_ROWS * MLP_LDW];
+  __shared__ float out[MLP_ROWS * NOUT];
+  const long base = (long)blockIdx.x * MLP_ROWS;
+  const long n = (long)*count;
+  const int t = threadIdx.x;
+  const float* in_mean = w.norm;
+  const float* in_std = w.norm + 64;
+  const float* out_mean = w.norm + 128;
+  const float* out_std = w.norm + 192;
+  if (t < MLP_ROWS) {
+    const long idx = base + t;
+    unsigned short* row = &h0[t * MLP_LDW];
+    for (int k = 0; k < KIN; ++k) row[k] = 0;
+    if (idx < n) {
+      const WS wc = cell_ws<M>(a.ws, list[idx]);
+      double rho = 0.0, sw = 0.0;
+      for (int k = 0; k < N; ++k) {
+        rho += wc[S::TMP + k];
+        sw += wc[S::TMP + k] * M::imw(k);
+      }
+      const double T = wc[S::TMP + N];
+      const double h = wc[S::LANE + H_WORD];
+      const double irho = 1.0 / rho;
+      double x = T;
+      row[0] = f2bf((float)((x - in_mean[0]) / in_std[0]));
+      x = log(M::RU * T * sw);                     // p = rho R T sum(Y_k / W_k)
+      row[1] = f2bf((float)((x - in_mean[1]) / in_std[1]));
+      for (int k = 0; k < N; ++k) {
+        x = log10(fmax(wc[S::TMP + k] * irho, 1e-20));
+        row[2 + k] = f2bf((float)((x - in_mean[2 + k]) / in_std[2 + k]));
+      }
+      x = log(h);
+      row[N + 2] = f2bf((float)((x - in_mean[N + 2]) / in_std[N + 2]));
+    }
+  }
+  __syncthreads();
+  unsigned short* hin = h0;
+  unsigned short* hout = h1;
+#pragma unroll 1
+  for (int l = 0; l < 3; ++l) {
+    auto emit = [&](int r, int c, float z) { hout[r * MLP_LDW + c] = f2bf(mlp_gelu(z)); };
+    if (l == 0)
+      mlp_dense<KIN, MLP_HIDDEN>(hin, w.wt[0], w.b[0], emit);
+    else
+      mlp_dense<MLP_HIDDEN, MLP_HIDDEN>(hin, w.wt[l], w.b[l], emit);
+    __syncthreads();
+    unsigned short* tmp = hin;
+    hin = hout;
+    hout = tmp;
+  }
+  mlp_dense<MLP_HIDDEN, NOUT>(hin, w.wt[3], w.b[3], [&](int r, int c, float z) { out[r * NOUT + c] = z; });
+  __syncthreads();
+  if (t < MLP_ROWS && base + t < n) {
+    const long c = list[base + t];
+    const WS wc = cell_ws<M>(a.ws, c);
+    Lane<M> st;
+    load_lane(st, wc, cell_iws<M>(a.iws, c), a);
+    if (st.pred_state == 1) {
+      double rho = 0.0;
+      for (int k = 0; k < N; ++k) rho += wc[S::TMP + k];
+      double g[N1];
+      bool finite = true;
+      for (int i = 0; i < N1; ++i) {
+        const double d = (double)out_mean[i] + (double)out_std[i] * (double)out[t * NOUT + i];
+        // output 0 is dT, outputs 1..N are dY_k
+        g[i] = (i == 0) ? wc[S::TMP + N] + d : 0.0;
+        if (i > 0) g[i] = wc[S::TMP + i - 1] + rho * d;
+        finite = finite && isfinite(g[i]);
+      }
+      if (finite) {
+        // state order is (rho Y_0 .. rho Y_{N-1}, T)
+        for (int k = 0; k < N; ++k) wc[S::ACOR + k] = g[k + 1] - st.zn(0, k);
+        wc[S::ACOR + N] = g[0] - st.zn(0, N);
+        st.a2 = 1.0;
+        st.pred_state = 2;
+        ++st.npred;
+      } else {
+        st.pred_state = 3;
+      }
+      store_lane(st, wc);
+    }
+  }
 }
 
 // LU of M = I - gamma J for every cell whose set-up is pending, one lane per cell
@@ -1146,9 +1276,9 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
   using S = Slots<N1>;
   const long idx = (long)blockIdx.x * 64 + threadIdx.x;
   const long n = (long)*count;
-  bool keep = false, want_setup = false;
+  bool keep = false, want_setup = false, want_pred = false;
   unsigned long long t_cells = 0, t_nst = 0, t_nfe = 0, t_nje = 0, t_nsetups = 0, t_nni = 0, t_ncfn = 0,
-                     t_netf = 0, t_fail = 0, t_maxst = 0, t_ir = 0, t_lufb = 0;
+                     t_netf = 0, t_fail = 0, t_maxst = 0, t_ir = 0, t_lufb = 0, t_pred = 0, t_predfb = 0;
   long c = -1;
   if (idx < n) {
     c = list[idx];
@@ -1212,10 +1342,11 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
         t_cells = 1;
         t_nst = st.nst; t_nfe = st.nfe; t_nje = st.nje; t_nsetups = st.nsetups; t_nni = st.nni;
         t_ncfn = st.ncfn; t_netf = st.netf; t_fail = ok ? 0 : 1; t_maxst = st.nst;
-        t_ir = st.nir; t_lufb = st.nlufb;
+        t_ir = st.nir; t_lufb = st.nlufb; t_pred = st.npred; t_predfb = st.npredfb;
       } else {
         keep = true;
         want_setup = st.needs_setup();
+        want_pred = (st.pred_state == 1);
       }
       store_lane(st, w);
     }
@@ -1223,6 +1354,7 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
   if (a.compact) {
     wave_append(keep, (int)c, next, next_count);
     wave_append(want_setup, (int)c, snext, snext_count);
+    if (a.use_pred) wave_append(want_pred, (int)c, a.plist, &a.counts[7]);
   } else {
     // fixed cell-to-lane mapping: only count the cells still integrating
     const unsigned long long b = __ballot(keep);
@@ -1241,6 +1373,8 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
     wave_max(&a.counters[CNT_MAXSTEPS], t_maxst);
     wave_add(&a.counters[CNT_IR], t_ir);
     wave_add(&a.counters[CNT_LUFALLBACK], t_lufb);
+    wave_add(&a.counters[CNT_PRED], t_pred);
+    wave_add(&a.counters[CNT_PREDFB], t_predfb);
   }
 }
 
@@ -1410,8 +1544,20 @@ struct Timer {
 };
 
 template <class M>
-hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* trips_out, KernelProf* prof) {
+hipError_t launch_predict(const ReactArgs& a, const MlpDev& w, unsigned np, hipStream_t stream) {
+  constexpr int KIN = (M::NS + 3 <= 32) ? 32 : 64;
+  constexpr int NOUT = (M::N1 <= 32) ? 32 : 64;
+  if (w.n_in != M::NS + 3 || w.n_out != M::N1 || w.kp[0] != KIN || w.np[3] != NOUT) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_predict<M, KIN, NOUT>), dim3((np + MLP_ROWS - 1) / MLP_ROWS), dim3(MLP_THREADS), 0, stream, a,
+                     w, a.plist, &a.counts[7]);
+  return hipGetLastError();
+}
+
+template <class M>
+hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* trips_out, KernelProf* prof,
+                       const MlpDev* pred) {
   const long n = a.ncell;
+  if (a.use_pred && (!pred || !a.compact || !a.plist)) return hipErrorInvalidValue;
   Timer* tmp = new Timer();
   tmp->prof = prof;
   tmp->stream = stream;
@@ -1445,6 +1591,14 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
       hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, nxt, 2 + nxt);
       PELE_TRY(hipGetLastError());
       const unsigned ga = (active + 63) / 64;
+      if (a.use_pred && h_counts[7] > 0) {
+        const unsigned np = h_counts[7];
+        PELE_TRY(tm.begin());
+        PELE_TRY(launch_predict<M>(a, *pred, np, stream));
+        PELE_TRY(tm.end(KK_PRED, np));
+        hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, 7, 7);
+        PELE_TRY(hipGetLastError());
+      }
       PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_eval<M>, dim3(ga), dim3(64), 0, stream, a, a.list[cur], &a.counts[cur]);
       PELE_TRY(hipGetLastError());
@@ -1548,10 +1702,10 @@ long iws_slots(int mech) {
 }
 
 hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out,
-                     KernelProf* prof) {
+                     KernelProf* prof, const MlpDev* pred) {
   switch (mech) {
-    case MECH_LIDRYER: return run_react_t<mech_LiDryer>(a, stream, max_trips, trips_out, prof);
-    case MECH_DRM19: return run_react_t<mech_drm19>(a, stream, max_trips, trips_out, prof);
+    case MECH_LIDRYER: return run_react_t<mech_LiDryer>(a, stream, max_trips, trips_out, prof, pred);
+    case MECH_DRM19: return run_react_t<mech_drm19>(a, stream, max_trips, trips_out, prof, pred);
     default: return hipErrorInvalidValue;
   }
 }

@@ -51,6 +51,9 @@ struct ReactArgs {
   int compact;                           // 1: finished cells leave the lists
   int* list[2];
   int* slist[2];
+  int* plist;                            // cells whose next attempt awaits the learned prediction
+  int use_pred;                          // 1: the learned predictor supplies each Newton attempt's
+                                         // initial iterate (compacted schedule only)
   unsigned int* counts;                  // 8 counters, see run_react
   // outputs
   unsigned long long* counters;          // per-call aggregate counters (see CounterIndex)
@@ -59,7 +62,7 @@ struct ReactArgs {
 
 enum CounterIndex : int {
   CNT_CELLS = 0, CNT_NST, CNT_NFE, CNT_NJE, CNT_NSETUPS, CNT_NNI, CNT_NCFN, CNT_NETF, CNT_NFAIL,
-  CNT_MAXSTEPS, CNT_IR, CNT_LUFALLBACK, CNT_HY2T, NCOUNTERS = 16
+  CNT_MAXSTEPS, CNT_IR, CNT_LUFALLBACK, CNT_HY2T, CNT_PRED, CNT_PREDFB, NCOUNTERS = 16
 };
 
 // Mechanism ids (order of the dispatch table in reactor_bdf.hip).
@@ -77,14 +80,16 @@ long iws_slots(int mech);       // ints per lane
 // Optional per-kernel timing: when `prof` is non-null every launch of the trip
 // kernels is bracketed by events and the per-kind device time (ms), launch count and
 // cells processed are accumulated into prof[kind].
-enum KernelKind : int { KK_EVAL = 0, KK_SETUP, KK_CONSUME, KK_LU, NKK };
+enum KernelKind : int { KK_EVAL = 0, KK_SETUP, KK_CONSUME, KK_LU, KK_PRED, NKK };
 struct KernelProf {
   double ms[NKK];
   long launches[NKK];
   long cells[NKK];
 };
+struct MlpDev;
+// pred: the learned predictor's device view, required when a.use_pred = 1
 hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out,
-                     KernelProf* prof);
+                     KernelProf* prof, const MlpDev* pred = nullptr);
 hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t stream);
 hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T, const int* mask, double* rYdot,
                         long ncell, hipStream_t stream);

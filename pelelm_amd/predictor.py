@@ -34,16 +34,25 @@ def _bind():
     L.mi_mlp_last_error.argtypes = [_vp]
     L.mi_mlp_last_error.restype = C.c_char_p
     L.mi_mlp_close.argtypes = [_vp]
+    L.mi_chem_attach_predictor.argtypes = [_vp, _vp]
     for f in ("mi_mlp_create", "mi_mlp_nparams", "mi_mlp_get_params", "mi_mlp_set_params",
-              "mi_mlp_set_normalisation", "mi_mlp_forward", "mi_mlp_grad", "mi_mlp_train_step", "mi_mlp_close"):
+              "mi_mlp_set_normalisation", "mi_mlp_forward", "mi_mlp_grad", "mi_mlp_train_step", "mi_mlp_close",
+              "mi_chem_attach_predictor"):
         getattr(L, f).restype = C.c_int
     L._mlp_bound = True
     return L
 
 
-MLP_EXPORTS = ["mi_mlp_create", "mi_mlp_nparams", "mi_mlp_get_params", "mi_mlp_set_params",
-               "mi_mlp_set_normalisation", "mi_mlp_forward", "mi_mlp_grad", "mi_mlp_train_step",
-               "mi_mlp_last_error", "mi_mlp_close"]
+MLP_EXPORTS = ["mi_chem_attach_predictor", "mi_mlp_close", "mi_mlp_create", "mi_mlp_forward", "mi_mlp_get_params",
+               "mi_mlp_grad", "mi_mlp_last_error", "mi_mlp_nparams", "mi_mlp_set_normalisation",
+               "mi_mlp_set_params", "mi_mlp_train_step"]
+
+
+def attach(reactor, net) -> None:
+    """Make ``net`` the Newton predictor of ``reactor`` (a ``ReactorBase``); None detaches."""
+    rc = _bind().mi_chem_attach_predictor(reactor._h, net.handle if net is not None else None)
+    if rc != 0:
+        raise MlpError(f"{rc}: {lib().mi_chem_last_error(reactor._h).decode()}")
 
 
 class MlpError(RuntimeError):
@@ -160,22 +169,88 @@ def train(net: Mlp, X, T, epochs: int = 10, batch: int = 8192, lr: float = 1e-3,
 
 
 # ---------------------------------------------------------------------------
-# reactor features: (T, log p, log10 Y_k, log dt) -> (dT, dY_k)
+# reactor features: (T, log p, log10 Y_k, log h) -> (dT, dY_k); the same definitions as
+# the reactor's predictor kernel (p = rho R T sum_k Y_k / W_k, Y floored at 1e-20)
 # ---------------------------------------------------------------------------
 Y_FLOOR = 1e-20
 
 
-def features(rY: np.ndarray, T: np.ndarray, dt, p: float = 1.01325e6) -> np.ndarray:
-    """Raw (unnormalised) inputs [ncell][N + 3] from SoA rY [N][ncell], T, dt."""
+def _mech_W(mech: str) -> np.ndarray:
+    from .mechanisms import mechanism
+    return np.array(mechanism.load(mech).W)
+
+
+def features(mech: str, rY: np.ndarray, T: np.ndarray, h) -> np.ndarray:
+    """Raw (unnormalised) inputs [ncell][N + 3] from SoA rY [N][ncell], T, step h."""
+    from .mechanisms import mechanism
+    W = _mech_W(mech)
     rho = rY.sum(axis=0)
     Y = rY / rho
     n = T.shape[0]
-    dt = np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,))
-    return np.concatenate([T[:, None], np.full((n, 1), math.log(p)), np.log10(np.maximum(Y, Y_FLOOR)).T,
-                           np.log(dt)[:, None]], axis=1)
+    h = np.broadcast_to(np.asarray(h, dtype=np.float64), (n,))
+    p = rho * mechanism.R_CGS * T * (Y / W[:, None]).sum(axis=0)
+    return np.concatenate([T[:, None], np.log(p)[:, None], np.log10(np.maximum(Y, Y_FLOOR)).T,
+                           np.log(h)[:, None]], axis=1)
 
 
 def targets(rY0, T0, rY1, T1) -> np.ndarray:
     """Raw outputs [ncell][N + 1]: (T1 - T0, Y1 - Y0) with Y = rY / rho (rho is constant)."""
     rho = rY0.sum(axis=0)
     return np.concatenate([(T1 - T0)[:, None], ((rY1 - rY0) / rho).T], axis=1)
+
+
+def generate_pairs(workload: str = "drm19_128_mix", n: int = 1_000_000, seed: int = 9104, groups: int = 32,
+                   h_range=(1e-10, 1e-5), tau_range=(1e-9, 1e-5), rtol: float = 1e-6, atol: float = 1e-10,
+                   log=None):
+    """Training pairs from the plain FP64 integrator on the GPU: seeded cells of the
+    workload's distribution (a seed of their own) are first advanced by a random time
+    tau (so intermediate integrator states are represented), then by a step h; the pair
+    is (features of the state after tau, h) -> increment over h.  Cells form `groups`
+    groups, each with its own log-uniform tau and h.  Returns (X, T) raw numpy arrays."""
+    import torch
+
+    from . import cells
+    from .reactor import ReactorBase
+    w = cells.WORKLOADS[workload]
+    d = cells.generate(w.mech, w.fuel, w.o2_per_fuel, w.phi, w.mode, seed, n)
+    rng = np.random.default_rng(seed)
+    taus = np.exp(rng.uniform(math.log(tau_range[0]), math.log(tau_range[1]), groups))
+    hs = np.exp(rng.uniform(math.log(h_range[0]), math.log(h_range[1]), groups))
+    r = ReactorBase.create("ReactorCvode")
+    r.set("mi.mechanism", w.mech)
+    r.set("ode.rtol", rtol)
+    r.set("ode.atol", atol)
+    r.init(2, 1)
+    m = n // groups
+    X, Tt = [], []
+    for g in range(groups):
+        sl = slice(g * m, (g + 1) * m)
+        y = torch.tensor(np.ascontiguousarray(d["rY"][:, sl]), device="cuda")
+        t = torch.tensor(d["T"][sl], device="cuda")
+        e = torch.tensor(d["rE"][sl], device="cuda")
+        fc = torch.zeros(m, dtype=torch.float64, device="cuda")
+        r.react(y, None, t, e, None, fc, None, float(taus[g]), 0.0)
+        y0, t0 = y.clone(), t.clone()
+        r.react(y, None, t, e, None, fc, None, float(hs[g]), 0.0)
+        torch.cuda.synchronize()
+        y0n, t0n, y1n, t1n = y0.cpu().numpy(), t0.cpu().numpy(), y.cpu().numpy(), t.cpu().numpy()
+        X.append(features(w.mech, y0n, t0n, hs[g]))
+        Tt.append(targets(y0n, t0n, y1n, t1n))
+        if log and (g + 1) % 8 == 0:
+            log(f"pairs: group {g + 1}/{groups} (tau {taus[g]:.2e} s, h {hs[g]:.2e} s)")
+    r.close()
+    return np.concatenate(X), np.concatenate(Tt)
+
+
+def fit(mech: str, X: np.ndarray, T: np.ndarray, epochs: int = 10, batch: int = 8192, lr: float = 2e-3,
+        lr_final: float = 1e-4, seed: int = 1, log=None) -> Mlp:
+    """Normalise the pairs, train a network on the GPU and set its normalisation."""
+    import torch
+    in_mean, in_std = X.mean(0), np.maximum(X.std(0), 1e-3)
+    out_mean, out_std = T.mean(0), np.maximum(T.std(0), 1e-30)
+    Xn = torch.tensor(((X - in_mean) / in_std).astype(np.float32), device="cuda")
+    Tn = torch.tensor(((T - out_mean) / out_std).astype(np.float32), device="cuda")
+    net = Mlp(X.shape[1], T.shape[1], seed=seed)
+    net.losses = train(net, Xn, Tn, epochs=epochs, batch=batch, lr=lr, lr_final=lr_final, seed=seed, log=log)
+    net.set_normalisation(in_mean, in_std, out_mean, out_std)
+    return net

@@ -1,5 +1,6 @@
 // Host stand-in for <hip/hip_runtime.h> used only to run the reactor's device code
-// serially on the CPU (one lane at a time) in tests.  Cross-lane operations assume a
+// serially on the CPU (one lane at a time) in tests; built with clang++ (the device
+// code uses clang vector extensions).  Cross-lane operations assume a
 // single active lane, so only the one-thread-per-cell schedule is meaningful here.
 #pragma once
 #include <cmath>
@@ -47,6 +48,9 @@ template <class T> inline T __shfl_down(T v, int, int = 64) { return T(0); }
 template <class T> inline T __shfl(T v, int, int = 64) { return v; }
 template <class T> inline T __shfl_xor(T v, int, int = 64) { return v; }
 inline void __syncthreads() {}
+// matrix-core products need all 64 lanes at once: kernels that use them (the learned
+// predictor) are compiled here but never launched
+#define __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, x, y, z) (c)
 #define __shared__ static
 inline unsigned long long __ballot(int p) { return p ? (1ull << __lane_id()) : 0ull; }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 def test_library_exports_every_network_symbol():
     from pelelm_amd import predictor
     text = open(os.path.join(ROOT, "include", "pelelm_mlp.h")).read()
-    syms = sorted(set(re.findall(r"\b(mi_mlp_\w+)\s*\(", text)))
+    syms = sorted(set(re.findall(r"\b(mi_(?:mlp|chem)_\w+)\s*\(", text)))
     lib = reactor.lib()
     for s in syms:
         assert hasattr(lib, s), s

@@ -1,0 +1,70 @@
+"""The learned Newton predictor inside the reactor (chosen form, C5 on drm19).
+
+A network is trained on the GPU on pairs the plain integrator generates (a seed of
+their own, not the evaluation cells), attached to the reactor, and the reactor is run
+on the drm19 stiffness-mix cells three ways: without a predictor, with the trained
+network and with an untrained one.  The error test is unchanged by the predictor, so
+every run must match the oracle to the plain form's tolerance; what the trained
+network gains is the Newton work (iterations, RHS evaluations) it saves, which the
+test prints and requires to be no worse than an untrained network's."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O  # noqa: E402
+from pelelm_amd import cells, predictor, reactor  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(d, net):
+    n = d["T"].shape[0]
+    r = reactor.ReactorBase.create("ReactorCvode")
+    r.set("mi.mechanism", "drm19")
+    r.set("ode.rtol", 1e-6)
+    r.set("ode.atol", 1e-10)
+    r.init(2, 1)
+    if net is not None:
+        predictor.attach(r, net)
+    rY = torch.tensor(d["rY"], device="cuda")
+    T = torch.tensor(d["T"], device="cuda")
+    rE = torch.tensor(d["rE"], device="cuda")
+    FC = torch.zeros(n, dtype=torch.float64, device="cuda")
+    r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    return rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), st
+
+
+def test_trained_predictor_keeps_parity_and_reports_its_gain():
+    print("generating 131072 training pairs with the plain integrator", flush=True)
+    X, Tt = predictor.generate_pairs("drm19_128_mix", n=131072, seed=9104, groups=16)
+    print("training", flush=True)
+    net = predictor.fit("drm19", X, Tt, epochs=8, batch=4096)
+    assert net.losses[-1] < net.losses[0]
+    untrained = predictor.Mlp(24, 22, seed=77)
+    untrained.set_normalisation(*net.norm)
+
+    d = cells.generate_workload("drm19_128_mix", ncell=8192)
+    ref = O.Mech("drm19").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    res = {}
+    for name, nn in (("plain", None), ("trained", net), ("untrained", untrained)):
+        print(f"reacting 8192 cells: {name}", flush=True)
+        gY, gT, gFC, st = _run(d, nn)
+        assert st["nfail"] == 0, name
+        assert (np.abs(gY - ref["rY"]) / tol).max() < 10.0, name
+        assert np.abs(gT - ref["T"]).max() < 1e-2, name
+        res[name] = st
+    for name, st in res.items():
+        print(f"{name:9s}: steps {st['nsteps']}, Newton iterations {st['nni']}, RHS {st['nfe']}, "
+              f"set-ups {st['nsetups']}, predictions used {st['npred_used']}, fallbacks {st['npred_fallback']}")
+    assert res["plain"]["npred_used"] == 0
+    assert res["trained"]["npred_used"] > 0
+    # training must pay off against a network that knows nothing
+    assert res["trained"]["nni"] <= res["untrained"]["nni"]
+    assert res["trained"]["npred_fallback"] <= res["untrained"]["npred_fallback"]
+    net.close()
+    untrained.close()

@@ -21,7 +21,9 @@ EMU = os.path.join(HERE, "cpu_emulation")
 LIB = os.path.join(EMU, "_build", "libemu.so")
 SRC = [os.path.join(EMU, "emu.cpp"), os.path.join(EMU, "hip", "hip_runtime.h"),
        os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "reactor_bdf.hip"),
-       os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "reactor_internal.h")]
+       os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "reactor_internal.h"),
+       os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "mlp_internal.h")]
+CLANG = "/opt/rocm/llvm/bin/clang++"   # host compile; the device code uses clang vector types
 
 
 @pytest.fixture(scope="module")
@@ -29,7 +31,7 @@ def emu():
     stale = not os.path.exists(LIB) or any(os.path.getmtime(s) > os.path.getmtime(LIB) for s in SRC)
     if stale:
         os.makedirs(os.path.dirname(LIB), exist_ok=True)
-        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + EMU, "-o", LIB, SRC[0]], check=True)
+        subprocess.run([CLANG, "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + EMU, "-o", LIB, SRC[0]], check=True)
     L = C.CDLL(LIB)
     L.emu_react.restype = C.c_int
     return L
