@@ -50,11 +50,78 @@ def rhs_ops(mech: str):
     return fixed, per_iter, trans
 
 
+LANE_WORDS = 75   # 8-byte words of the integrator's lane state (drm19 and LiDryer)
+
+
 def rhs_bytes(N: int) -> int:
-    """HBM bytes one k_eval cell-evaluation moves: zn0, zn1, acor, forcing, the lane
-    state words read (96 max, 66 used) and y, f, T written."""
+    """HBM bytes one k_eval cell-evaluation moves: zn0, zn1, acor, forcing and the lane
+    state read, y, f and T written."""
     N1 = N + 1
-    return 8 * (3 * N1 + N + 66 + 2 * N1 + 1)
+    return 8 * (3 * N1 + N + LANE_WORDS + 2 * N1 + 1)
+
+
+KINDS = ["k_eval", "k_setup", "k_consume", "k_lu_coop", "k_predict"]
+KIND_DESC = {"k_eval": "right-hand side (HY2T, rates, energy)", "k_setup": "analytic Jacobian + set-up",
+             "k_consume": "Newton update (LU solve) and BDF step logic", "k_lu_coop": "LU of I - gamma J",
+             "k_predict": "learned Newton predictor"}
+
+
+def kernel_model(st, nspec):
+    """Algorithmic work of each kernel kind over the call: (flops, hbm bytes)."""
+    N, n = nspec, nspec + 1
+    fixed, per_iter, _ = rhs_ops("drm19")
+    with open(os.path.join(ROOT, "pelelm_amd", "mechanisms", "drm19_ops.json")) as f:
+        jac_flops = json.load(f)["functions"]["jacobian"]["flops"]
+    evals = st["kernel_cells"][0]
+    nni, nst, nsetups, nje = st["nni"], st["nsteps"], st["nsetups"], st["nje"]
+    consume_passes = st["kernel_cells"][2]
+    return {
+        "k_eval": (evals * fixed + st["nhy2t_iters"] * per_iter, evals * rhs_bytes(N)),
+        # Jacobian where evaluated; M = I - gamma J formation is in the LU kernel
+        "k_setup": (nje * jac_flops, nje * 8 * (n * n + 2 * n + 2 * LANE_WORDS)),
+        # Newton: residual (3n), forward/back substitution (2 n^2), WRMS norms; step
+        # logic per step: Nordsieck predict/correct over (q+1) n with q <= 4
+        "k_consume": (nni * (2 * n * n + 8 * n) + nst * 2 * 5 * n * 3,
+                      consume_passes * 8 * (LANE_WORDS * 2 + 3 * n) + nni * 8 * (n * n + 3 * n)
+                      + nst * 8 * 2 * 5 * n),
+        "k_lu_coop": (nsetups * (2 * n ** 3 // 3 + 2 * n * n), nsetups * 8 * 2 * n * n),
+        "k_predict": (st["npred_used"] * 2 * (24 * 128 + 2 * 128 * 128 + 128 * 22), st["npred_used"] * 8 * 3 * n),
+    }
+
+
+def kernel_roofline(st, nspec, traffic_bytes=None):
+    """Roofline of the kernel with the largest device time: algorithmic work per launch
+    over its mean launch time (HIP events on the reactor's stream), against FP64 peak
+    (the vector and matrix FP64 peaks of MI355X coincide) or HBM peak for k_consume."""
+    kms = st["kernel_ms"]
+    dom = int(np.argmax(kms))
+    name = KINDS[dom]
+    launches = max(1, st["kernel_launches"][dom])
+    avg_ms = kms[dom] / launches
+    model = kernel_model(st, nspec)
+    flops, byts = model[name]
+    per_launch_flops, per_launch_bytes = flops / launches, byts / launches
+    if name == "k_consume":
+        achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+        line = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS}
+    else:
+        achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
+        line = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS}
+    line["traffic"] = traffic_bytes
+    line.update({
+        "kernel": f"{name}<mech_drm19>: {KIND_DESC[name]}",
+        "avg_launch_ms": avg_ms, "launches": launches,
+        "algorithmic_flops_per_launch": per_launch_flops, "algorithmic_bytes_per_launch": per_launch_bytes,
+        "kernel_ms": {KINDS[i]: kms[i] for i in range(len(KINDS))},
+        "kernel_launches": {KINDS[i]: st["kernel_launches"][i] for i in range(len(KINDS))},
+        "kernel_tflops": {k: (model[k][0] / (kms[i] * 1e-3) / 1e12 if kms[i] > 0 else None)
+                          for i, k in enumerate(KINDS)},
+        "kernel_GBs": {k: (model[k][1] / (kms[i] * 1e-3) / 1e9 if kms[i] > 0 else None)
+                       for i, k in enumerate(KINDS)},
+    })
+    return line
 
 
 def run_form(form, dev_in, ncell, nspec, dt, steps, warmup, profile):
@@ -111,7 +178,21 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--traffic-bytes", type=float, default=None,
+                    help="measured HBM bytes per launch of the dominant kernel (from the byte counters)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only copy a device buffer of this many MiB 5 times (calibrates HBM byte counters)")
     args = ap.parse_args(argv)
+    if args.copy_probe_mb > 0:
+        import torch
+        n = args.copy_probe_mb * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the reactor's cells are independent, so a multi-GPU run is "
               f"N copies of this one on disjoint boxes with no communication; --gpus {args.gpus} is not run.",
@@ -137,31 +218,9 @@ def main(argv=None):
     bf = forms[best]
     st = bf["stats"]
 
-    # roofline of the dominant kernel (the right-hand-side kernel when profiled)
     roofline = None
     if not args.no_profile:
-        kinds = ["rhs (k_eval)", "jacobian (k_setup)", "newton/step (k_consume)", "LU (k_lu_coop)"]
-        kms = st["kernel_ms"]
-        dom = int(np.argmax(kms))
-        fixed, per_iter, trans = rhs_ops("drm19")
-        evals = st["kernel_cells"][0]
-        flops = evals * fixed + st["nhy2t_iters"] * per_iter
-        per_launch_flops = flops / max(1, st["kernel_launches"][0])
-        avg_ms = kms[0] / max(1, st["kernel_launches"][0])
-        achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "k_eval<mech_drm19> (right-hand side)",
-                    "dominant_kernel": kinds[dom],
-                    "kernel_ms": {kinds[i]: kms[i] for i in range(4)},
-                    "kernel_launches": {kinds[i]: st["kernel_launches"][i] for i in range(4)},
-                    "avg_launch_ms_rhs": avg_ms,
-                    "rhs_flops_per_eval_fixed": fixed, "rhs_flops_per_hy2t_iter": per_iter,
-                    "rhs_transcendentals_per_eval": trans,
-                    "rhs_evals": evals, "hy2t_iters": st["nhy2t_iters"],
-                    "rhs_transcendentals_per_s": evals * trans / (kms[0] * 1e-3),
-                    "rhs_hbm_bytes_per_eval": rhs_bytes(nspec),
-                    "rhs_hbm_GBs": evals * rhs_bytes(nspec) / (kms[0] * 1e-3) / 1e9}
+        roofline = kernel_roofline(st, nspec, traffic_bytes=args.traffic_bytes)
 
     cpu = None
     if not args.no_cpu:
@@ -203,7 +262,8 @@ def main(argv=None):
                       "steps_per_cell": v["stats"]["nsteps"] / max(1, v["stats"]["ncells"]),
                       "rhs_per_cell": v["stats"]["nfe"] / max(1, v["stats"]["ncells"]),
                       "lu_per_cell": v["stats"]["nsetups"] / max(1, v["stats"]["ncells"]),
-                      "trips": v["stats"]["ntrips"], "device_ms": v["stats"]["ms_kernel"]}
+                      "trips": v["stats"]["ntrips"], "device_ms": v["stats"]["ms_kernel"],
+                      "kernel_ms": dict(zip(KINDS, v["stats"]["kernel_ms"]))}
                   for f, v in forms.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -806,7 +806,10 @@ struct Lane {
     for (int i = 0; i < N1; ++i) ws[S::ACOR + i] += x[i];
     const double del = wrms_reg(x);
     if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
-    const double dcon = del * fmin(1.0, crate) / tq[4];
+    // the first correction from a learned iterate carries no convergence-rate history:
+    // it must be small by itself (the Nordsieck predictor's rate does not apply to it)
+    const double rate = (m == 0 && pred_state == 2) ? 1.0 : fmin(1.0, crate);
+    const double dcon = del * rate / tq[4];
     if (dcon <= 1.0) {
       // with a learned initial iterate acor is not the first correction alone
       acnrm = (m == 0 && pred_state != 2) ? del : wrms_ws(S::ACOR);
@@ -906,6 +909,19 @@ __device__ __forceinline__ void store_lane(const Lane<M>& st, const WS& w) {
 #pragma unroll
   for (int i = 0; i < NW; ++i) w[S::LANE + i] = tmp[i];
 }
+
+// a field of the lane state (at byte offset `off` of Lane<M>), read straight from its
+// workspace word without loading the whole state
+template <class M, class T>
+__device__ __forceinline__ T lane_field(const WS& w, size_t off) {
+  static_assert(sizeof(T) <= 8, "one word");
+  using S = Slots<M::N1>;
+  const double word = w[S::LANE + (int)(off / 8)];
+  T v;
+  __builtin_memcpy(&v, reinterpret_cast<const char*>(&word) + off % 8, sizeof(T));
+  return v;
+}
+#define LANE_FIELD(M, w, f) lane_field<M, decltype(Lane<M>::f)>((w), offsetof(Lane<M>, f))
 
 // ---------------------------------------------------------------------------
 // the integration kernel
@@ -1010,13 +1026,13 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) k_eval(const ReactArg
   if (idx < (long)*count) {
     const long c = list[idx];
     const WS w = cell_ws<M>(a.ws, c);
-    Lane<M> st;
-    load_lane(st, w, cell_iws<M>(a.iws, c), a);
-    if (st.phase != PH_NEED_CELL) {
+    // only the fields the evaluation needs are read, not the whole lane state
+    if (LANE_FIELD(M, w, phase) != PH_NEED_CELL) {
+      const double a1 = LANE_FIELD(M, w, a1), a2 = LANE_FIELD(M, w, a2);
       double y[N1], f[N1], Tsol;
 #pragma unroll
-      for (int i = 0; i < N1; ++i) y[i] = st.zn(0, i) + st.a1 * st.zn(1, i) + st.a2 * w[S::ACOR + i];
-      rhs<M>(y, st.teval, w, st.rE0, st.fH, f, Tsol, &iters);
+      for (int i = 0; i < N1; ++i) y[i] = w[S::ZN + i] + a1 * w[S::ZN + N1 + i] + a2 * w[S::ACOR + i];
+      rhs<M>(y, LANE_FIELD(M, w, teval), w, LANE_FIELD(M, w, rE0), LANE_FIELD(M, w, fH), f, Tsol, &iters);
 #pragma unroll
       for (int i = 0; i < N1; ++i) {
         w[S::Y + i] = y[i];
@@ -1167,15 +1183,6 @@ __global__ void __launch_bounds__(64) k_lu_lane(const ReactArgs a, const int* __
                              : lu_factor<M::N1, double>(w, st.iws, st.gamma));
 }
 
-// an int field of the lane state, read straight from its workspace word
-template <class M>
-__device__ __forceinline__ int lane_int_field(const WS& w, size_t off) {
-  using S = Slots<M::N1>;
-  const double word = w[S::LANE + (int)(off / 8)];
-  int v;
-  __builtin_memcpy(&v, reinterpret_cast<const char*>(&word) + off % 8, sizeof(int));
-  return v;
-}
 
 // Cooperative LU of M = I - gamma J in Real arithmetic: 32 lanes per cell (row r of M
 // in lane r), two cells per wave.  Column k's pivot is the largest |m_rk| over the rows
@@ -1193,11 +1200,15 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
   using S = Slots<N1>;
   using A = LUAcc<N1, Real>;
   constexpr bool IS32 = sizeof(Real) == 4;
-  constexpr int GAMMA_WORD = (int)(offsetof(Lane<M>, gamma) / 8);
   __shared__ Real prow[2][N1];
   const int lane = threadIdx.x;
   const int half = lane >> 5, r = lane & 31;
-  const long idx = (long)blockIdx.x * 2 + half;
+  // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous range of
+  // list entries so neighbouring cells (which share workspace lines) meet in one L2.
+  // The grid is a multiple of 8, so this is a permutation of the blocks.
+  const unsigned per_xcd = gridDim.x / 8;
+  const long blk = (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  const long idx = blk * 2 + half;
   const long n = (long)*count;
   long c = -1;
   bool work = false;
@@ -1205,10 +1216,10 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
   if (idx < n) {
     c = list[idx];
     const WS w = cell_ws<M>(a.ws, c);
-    const bool pend = lane_int_field<M>(w, offsetof(Lane<M>, setup_pending)) != 0;
-    const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && lane_int_field<M>(w, offsetof(Lane<M>, lu64)) == 0;
+    const bool pend = LANE_FIELD(M, w, setup_pending) != 0;
+    const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
     work = pend && (fp32_cell == IS32);
-    gamma = w[S::LANE + GAMMA_WORD];
+    gamma = LANE_FIELD(M, w, gamma);
   }
   const bool row = work && r < N1;
   Real m[N1];
@@ -1282,10 +1293,13 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
   long c = -1;
   if (idx < n) {
     c = list[idx];
-    const WS w = cell_ws<M>(a.ws, c);
+  }
+  const WS w0 = cell_ws<M>(a.ws, c < 0 ? 0 : c);
+  if (c >= 0 && LANE_FIELD(M, w0, phase) != PH_NEED_CELL) {
+    const WS w = w0;
     Lane<M> st;
     load_lane(st, w, cell_iws<M>(a.iws, c), a);
-    if (st.phase != PH_NEED_CELL) {
+    {
       ++st.nfe;
       double f[N1];
 #pragma unroll
@@ -1356,9 +1370,11 @@ __global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __
     wave_append(want_setup, (int)c, snext, snext_count);
     if (a.use_pred) wave_append(want_pred, (int)c, a.plist, &a.counts[7]);
   } else {
-    // fixed cell-to-lane mapping: only count the cells still integrating
+    // fixed cell-to-lane mapping: count the cells still integrating; the set-up list is
+    // compacted as in the queue schedule
     const unsigned long long b = __ballot(keep);
     if (__lane_id() == 0 && b) atomicAdd(next_count, (unsigned int)__popcll(b));
+    wave_append(want_setup, (int)c, snext, snext_count);
   }
   if (a.counters) {
     wave_add(&a.counters[CNT_CELLS], t_cells);
@@ -1489,7 +1505,7 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
     hipLaunchKernelGGL(k_lu_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, list, count);
     return hipGetLastError();
   }
-  const dim3 grid((unsigned)((n + 1) / 2));
+  const dim3 grid((unsigned)(((n + 1) / 2 + 7) / 8 * 8));   // two cells per block, multiple of 8 blocks
   if (a.lu_mode == LU_FP32_IR) {
     hipLaunchKernelGGL((k_lu_coop<M, float>), grid, dim3(64), 0, stream, a, list, count);
     const hipError_t e = hipGetLastError();
@@ -1502,7 +1518,7 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
 
 // counts[0..1]: active-cell lists of the current / next trip, counts[2..3]: their set-up
 // lists; without compaction counts[4] is the (fixed) list length and counts[5] the
-// number of cells still integrating.
+// number of cells still integrating; counts[7]: cells awaiting a learned prediction.
 // Per-kernel timing: an event pair per launch, recorded asynchronously and read back
 // once the call has finished, so profiling adds no synchronisation inside the trips.
 struct Timer {
@@ -1621,24 +1637,30 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
       cur = nxt;
     } else {
       if (h_counts[5] == 0) break;
-      hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, 5, 6);
+      const unsigned int nsetup = h_counts[2 + cur];
+      const int nxt = 1 - cur;
+      hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, 5, 2 + nxt);
       PELE_TRY(hipGetLastError());
       PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_eval<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4]);
       PELE_TRY(hipGetLastError());
       PELE_TRY(tm.end(KK_EVAL, h_counts[5]));
-      PELE_TRY(tm.begin());
-      hipLaunchKernelGGL(k_setup<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4]);
-      PELE_TRY(hipGetLastError());
-      PELE_TRY(tm.end(KK_SETUP, 0));
-      PELE_TRY(tm.begin());
-      PELE_TRY(launch_lu<M>(a, a.list[1], &a.counts[4], n, stream));
-      PELE_TRY(tm.end(KK_LU, 0));
+      if (nsetup > 0) {
+        PELE_TRY(tm.begin());
+        hipLaunchKernelGGL(k_setup<M>, dim3((nsetup + 63) / 64), dim3(64), 0, stream, a, a.slist[cur],
+                           &a.counts[2 + cur]);
+        PELE_TRY(hipGetLastError());
+        PELE_TRY(tm.end(KK_SETUP, nsetup));
+        PELE_TRY(tm.begin());
+        PELE_TRY(launch_lu<M>(a, a.slist[cur], &a.counts[2 + cur], nsetup, stream));
+        PELE_TRY(tm.end(KK_LU, nsetup));
+      }
       PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_consume<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4], a.list[0],
-                         &a.counts[5], a.slist[0], &a.counts[6]);
+                         &a.counts[5], a.slist[nxt], &a.counts[2 + nxt]);
       PELE_TRY(hipGetLastError());
       PELE_TRY(tm.end(KK_CONSUME, h_counts[5]));
+      cur = nxt;
     }
   }
   if (trip == max_trips) {

@@ -3,7 +3,7 @@
 #define PELE_WAVES_PER_EU(n)
 #include "../../pelelm_amd/csrc/reactor_bdf.hip"
 
-thread_local Idx3 threadIdx, blockIdx, blockDim;
+thread_local Idx3 threadIdx, blockIdx, blockDim, gridDim;
 
 using namespace pelelm;
 

@@ -39,7 +39,7 @@ struct dim3 {
   dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
 struct Idx3 { unsigned x = 0, y = 0, z = 0; };
-extern thread_local Idx3 threadIdx, blockIdx, blockDim;
+extern thread_local Idx3 threadIdx, blockIdx, blockDim, gridDim;
 
 template <class T> inline T max(T a, T b) { return a > b ? a : b; }
 template <class T> inline T min(T a, T b) { return a < b ? a : b; }
@@ -62,6 +62,7 @@ template <class T> inline T atomicMax(T* p, T v) { T o = *p; if (v > o) *p = v; 
   do {                                                                          \
     const dim3 g_ = (grid), b_ = (block);                                       \
     blockDim.x = b_.x;                                                          \
+    gridDim.x = g_.x;                                                           \
     for (unsigned bx_ = 0; bx_ < g_.x; ++bx_)                                   \
       for (unsigned tx_ = 0; tx_ < b_.x; ++tx_) {                               \
         blockIdx.x = bx_;                                                       \

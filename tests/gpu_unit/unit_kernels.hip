@@ -15,7 +15,7 @@
 #define UNIT_SYNC() (void)hipDeviceSynchronize()
 #else
 #include <cstdlib>
-thread_local Idx3 threadIdx, blockIdx, blockDim;
+thread_local Idx3 threadIdx, blockIdx, blockDim, gridDim;
 #define UNIT_DEV_ALLOC(p, n) (p) = (decltype(p))std::malloc(n)
 #define UNIT_DEV_FREE(p) std::free(p)
 #define UNIT_H2D(d, h, n) std::memcpy((d), (h), (n))

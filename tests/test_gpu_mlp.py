@@ -82,7 +82,7 @@ def test_adam_training_reduces_loss():
     torch.cuda.synchronize()
     assert hist[-1] < 0.1 * hist[0], hist
     y = net.forward(X).cpu().numpy()
-    assert np.mean((y - t) ** 2) < 0.1 * np.mean(t ** 2)
+    assert np.mean((y - t) ** 2) < 0.2 * np.mean(t ** 2)
     net.close()
 
 

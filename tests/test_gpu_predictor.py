@@ -3,10 +3,12 @@
 A network is trained on the GPU on pairs the plain integrator generates (a seed of
 their own, not the evaluation cells), attached to the reactor, and the reactor is run
 on the drm19 stiffness-mix cells three ways: without a predictor, with the trained
-network and with an untrained one.  The error test is unchanged by the predictor, so
-every run must match the oracle to the plain form's tolerance; what the trained
-network gains is the Newton work (iterations, RHS evaluations) it saves, which the
-test prints and requires to be no worse than an untrained network's."""
+network and with an untrained one.  The error test is unchanged by the predictor, but
+the Newton iterates are not, so results differ from the plain form within the Newton
+tolerance: every run must stay within 1e-3 K of the FP64 integrator in T and be no
+less accurate than the plain form against the tight oracle (rtol 1e-12).  What the
+trained network gains is the Newton work (iterations, RHS evaluations) it saves, which
+the test prints and requires to be no worse than an untrained network's."""
 import numpy as np
 import pytest
 
@@ -48,19 +50,29 @@ def test_trained_predictor_keeps_parity_and_reports_its_gain():
     untrained.set_normalisation(*net.norm)
 
     d = cells.generate_workload("drm19_128_mix", ncell=8192)
-    ref = O.Mech("drm19").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
-    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
-    res = {}
+    m = O.Mech("drm19")
+    ref = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    tight = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-12, atol=1e-20, max_steps=200000)
+    tol = 1e-6 * np.abs(tight["rY"]) + 1e-10
+    res, err = {}, {}
     for name, nn in (("plain", None), ("trained", net), ("untrained", untrained)):
         print(f"reacting 8192 cells: {name}", flush=True)
         gY, gT, gFC, st = _run(d, nn)
-        assert st["nfail"] == 0, name
-        assert (np.abs(gY - ref["rY"]) / tol).max() < 10.0, name
-        assert np.abs(gT - ref["T"]).max() < 1e-2, name
         res[name] = st
+        err[name] = dict(dT_vs_fp64=np.abs(gT - ref["T"]).max(), dT_vs_tight=np.abs(gT - tight["T"]).max(),
+                         y_vs_tight=(np.abs(gY - tight["rY"]) / tol).max(), nfail=st["nfail"])
     for name, st in res.items():
         print(f"{name:9s}: steps {st['nsteps']}, Newton iterations {st['nni']}, RHS {st['nfe']}, "
-              f"set-ups {st['nsetups']}, predictions used {st['npred_used']}, fallbacks {st['npred_fallback']}")
+              f"set-ups {st['nsetups']}, predictions used {st['npred_used']}, fallbacks {st['npred_fallback']}; "
+              f"max|dT| vs FP64 oracle {err[name]['dT_vs_fp64']:.3g} K, vs tight oracle "
+              f"{err[name]['dT_vs_tight']:.3g} K, species error vs tight oracle {err[name]['y_vs_tight']:.3g} tol")
+    for name in res:
+        assert err[name]["nfail"] == 0, name
+        # the chosen form's bar: |dT| < 1e-3 K against the FP64 integrator at the same rtol
+        assert err[name]["dT_vs_fp64"] < 1e-3, name
+        # and no worse accuracy than the plain form against the tight oracle
+        assert err[name]["y_vs_tight"] <= 2.0 * err["plain"]["y_vs_tight"] + 1.0, name
+        assert err[name]["dT_vs_tight"] <= 2.0 * err["plain"]["dT_vs_tight"] + 1e-3, name
     assert res["plain"]["npred_used"] == 0
     assert res["trained"]["npred_used"] > 0
     # training must pay off against a network that knows nothing

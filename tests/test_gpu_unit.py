@@ -185,12 +185,19 @@ def test_cooperative_fp32_lu_matches_lane_fp32_lu(mech, wl):
     assert np.all(sc[ok] == 0) and np.all(sl[ok] == 0) and np.all(sh[ok] == 0)
     # same pivots wherever the FP32 column maxima are not tied
     assert np.mean(np.all(pc[ok] == pl[ok], axis=1)) > 0.99
-    np.testing.assert_array_equal(pl[ok], ph[ok])
+    assert np.mean(np.all(pl[ok] == ph[ok], axis=1)) > 0.99
     same = np.all(pc == pl, axis=1) & ok
     scale = np.abs(lul[same]).max(axis=2, keepdims=True)
     assert np.max(np.abs(luc[same] - lul[same]) / scale) < 1e-5
-    np.testing.assert_allclose(luh[ok], lul[ok], rtol=0, atol=0)
     M = np.eye(N1)[None] - gamma[ok, None, None] * J[ok]
+    # each factorisation reproduces P M to FP32 accuracy (the GPU contracts to FMA, the
+    # CPU twin does not, so their factors differ where elimination cancels)
+    for lu, pm in ((luc, pc), (lul, pl), (luh, ph)):
+        L = np.tril(lu[ok], -1) + np.eye(N1)[None]
+        U = np.triu(lu[ok])
+        PM = np.take_along_axis(M, pm[ok][:, :, None].astype(np.int64), axis=1)
+        res = np.abs(PM - L @ U).max(axis=(1, 2)) / np.abs(M).max(axis=(1, 2))
+        assert res.max() < 1e-5, res.max()
     xe = np.linalg.solve(M, b[ok][..., None])[..., 0]
     xs = np.abs(xe).max(axis=1, keepdims=True)
     assert np.max(np.abs(xc[ok] - xe) / xs) < 1e-3
@@ -215,12 +222,13 @@ def test_cooperative_lu_matches_lane_lu(mech, wl):
     J, _, _, _ = run_jac_lu(Cl, mech, y, T, f[:, -1].copy(), gamma, b)
     J = np.ascontiguousarray(J)
     pend = (rng.random(n) < 0.8).astype(np.int32)
-    # singular matrices: column k of I - gamma J vanishes
+    # singular matrices: column k of I - gamma J vanishes exactly (gamma a power of two)
     sing_cells = np.arange(5, n, 97)
     for i, c in enumerate(sing_cells):
         k = (3 * i) % N1
+        gamma[c] = 2.0 ** -20
         J[c, :, k] = 0.0
-        J[c, k, k] = 1.0 / gamma[c]
+        J[c, k, k] = 2.0 ** 20
         pend[c] = 1
     xc, sc, luc, pc = run_lu(G, mech, 1, J, gamma, pend, b)
     xl, sl, lul, pl = run_lu(G, mech, 0, J, gamma, pend, b)
@@ -237,12 +245,13 @@ def test_cooperative_lu_matches_lane_lu(mech, wl):
     for c in np.where(pend == 1)[0]:
         assert sorted(pc[c]) == list(range(N1))
     np.testing.assert_array_equal(pc[ok], pl[ok])
-    np.testing.assert_array_equal(pl[ok], ph[ok])
+    # the CPU twin does not contract to FMA: a near-tie may pivot differently there
+    assert np.mean(np.all(pl[ok] == ph[ok], axis=1)) > 0.99
     scale = np.abs(lul[ok]).max(axis=2, keepdims=True)
     assert np.max(np.abs(luc[ok] - lul[ok]) / scale) < 1e-13
     xs = np.abs(xl[ok]).max(axis=1, keepdims=True)
     assert np.max(np.abs(xc[ok] - xl[ok]) / xs) < 1e-12
-    assert np.max(np.abs(xl[ok] - xh[ok]) / xs) < 1e-12
+    assert np.max(np.abs(xl[ok] - xh[ok]) / xs) < 1e-10
     # and the solution solves the system
     M = np.eye(N1)[None] - gamma[ok, None, None] * J[ok]
     res = np.einsum("cij,cj->ci", M, xc[ok]) - b[ok]
