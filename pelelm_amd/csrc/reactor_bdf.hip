@@ -28,6 +28,14 @@
 #include "reactor_internal.h"
 #include "mlp_internal.h"
 
+// minimum resident waves per SIMD the heavy trip kernels are compiled for (the
+// register budget follows: 512 / n VGPRs per lane)
+#ifndef PELE_CONSUME_WAVES
+#define PELE_CONSUME_WAVES 1
+#endif
+#ifndef PELE_SETUP_WAVES
+#define PELE_SETUP_WAVES 1
+#endif
 #ifndef PELE_WAVES_PER_EU
 #define PELE_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
@@ -806,11 +814,11 @@ struct Lane {
     for (int i = 0; i < N1; ++i) ws[S::ACOR + i] += x[i];
     const double del = wrms_reg(x);
     if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
-    // the first correction from a learned iterate carries no convergence-rate history:
-    // it must be small by itself (the Nordsieck predictor's rate does not apply to it)
-    const double rate = (m == 0 && pred_state == 2) ? 1.0 : fmin(1.0, crate);
-    const double dcon = del * rate / tq[4];
-    if (dcon <= 1.0) {
+    const double dcon = del * fmin(1.0, crate) / tq[4];
+    // a learned iterate carries no convergence-rate history (the rate estimate belongs
+    // to the Nordsieck predictor), so its first correction is never taken as converged:
+    // convergence is judged from the second iteration on, with the rate it measures
+    if (dcon <= 1.0 && !(m == 0 && pred_state == 2)) {
       // with a learned initial iterate acor is not the first correction alone
       acnrm = (m == 0 && pred_state != 2) ? del : wrms_ws(S::ACOR);
       jcur = false;
@@ -1046,7 +1054,7 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) k_eval(const ReactArg
 
 // Jacobian (when stale) and LU of I - gamma J for the cells starting a Newton attempt
 template <class M>
-__global__ void __launch_bounds__(64) k_setup(const ReactArgs a, const int* __restrict__ list,
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_SETUP_WAVES) k_setup(const ReactArgs a, const int* __restrict__ list,
                                               const unsigned int* __restrict__ count) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
@@ -1280,7 +1288,7 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
 
 // advance every active cell's state machine by this trip's evaluation
 template <class M>
-__global__ void __launch_bounds__(64) k_consume(const ReactArgs a, const int* __restrict__ list,
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_consume(const ReactArgs a, const int* __restrict__ list,
                                                 const unsigned int* __restrict__ count, int* next,
                                                 unsigned int* next_count, int* snext, unsigned int* snext_count) {
   constexpr int N = M::NS, N1 = M::N1;
