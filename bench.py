@@ -60,10 +60,10 @@ def rhs_bytes(N: int) -> int:
     return 8 * (3 * N1 + N + LANE_WORDS + 2 * N1 + 1)
 
 
-KINDS = ["k_eval", "k_setup", "k_consume", "k_lu_coop", "k_predict"]
+KINDS = ["k_eval", "k_setup", "k_consume", "k_lu_coop", "k_predict", "k_solve_coop"]
 KIND_DESC = {"k_eval": "right-hand side (HY2T, rates, energy)", "k_setup": "analytic Jacobian + set-up",
              "k_consume": "Newton update (LU solve) and BDF step logic", "k_lu_coop": "LU of I - gamma J",
-             "k_predict": "learned Newton predictor"}
+             "k_predict": "learned Newton predictor", "k_solve_coop": "Newton linear solves (substitution)"}
 
 
 def kernel_model(st, nspec):
@@ -79,11 +79,12 @@ def kernel_model(st, nspec):
         "k_eval": (evals * fixed + st["nhy2t_iters"] * per_iter, evals * rhs_bytes(N)),
         # Jacobian where evaluated; M = I - gamma J formation is in the LU kernel
         "k_setup": (nje * jac_flops, nje * 8 * (n * n + 2 * n + 2 * LANE_WORDS)),
-        # Newton: residual (3n), forward/back substitution (2 n^2), WRMS norms; step
-        # logic per step: Nordsieck predict/correct over (q+1) n with q <= 4
-        "k_consume": (nni * (2 * n * n + 8 * n) + nst * 2 * 5 * n * 3,
-                      consume_passes * 8 * (LANE_WORDS * 2 + 3 * n) + nni * 8 * (n * n + 3 * n)
-                      + nst * 8 * 2 * 5 * n),
+        # Newton update, convergence test and WRMS norms; step logic per step: Nordsieck
+        # predict/correct over (q+1) n with q <= 4
+        "k_consume": (nni * 8 * n + nst * 2 * 5 * n * 3,
+                      consume_passes * 8 * (LANE_WORDS * 2 + 3 * n) + nni * 8 * 3 * n + nst * 8 * 2 * 5 * n),
+        # forward/back substitution with the factors (n^2 words) and B, X vectors
+        "k_solve_coop": (nni * 2 * n * n, nni * 8 * (n * n + 2 * n)),
         "k_lu_coop": (nsetups * (2 * n ** 3 // 3 + 2 * n * n), nsetups * 8 * 2 * n * n),
         "k_predict": (st["npred_used"] * 2 * (24 * 128 + 2 * 128 * 128 + 128 * 22), st["npred_used"] * 8 * 3 * n),
     }

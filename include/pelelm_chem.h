@@ -30,7 +30,7 @@ extern "C" {
 #define MI_ECELLFAIL -4   /* at least one cell failed to integrate          */
 #define MI_ENOMEM -5
 
-#define MI_NKERNEL_KINDS 5
+#define MI_NKERNEL_KINDS 6
 
 /* amrex::Array4<Real>-compatible view: element (i,j,k,n) lives at
    p[(i-begin[0]) + (j-begin[1])*jstride + (k-begin[2])*kstride + n*nstride]. */
@@ -65,7 +65,7 @@ typedef struct {
   long nhy2t_iters;     /* Newton iterations of the T-from-enthalpy solves      */
   /* with mi.profile = 1: device time (ms), launches and cells processed per kernel
      kind [0] right-hand side, [1] Jacobian set-up, [2] Newton/step logic, [3] LU,
-     [4] learned predictor */
+     [4] learned predictor, [5] Newton linear solves */
   double kernel_ms[MI_NKERNEL_KINDS];
   long kernel_launches[MI_NKERNEL_KINDS];
   long kernel_cells[MI_NKERNEL_KINDS];

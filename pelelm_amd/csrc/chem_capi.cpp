@@ -47,7 +47,8 @@ struct Reactor {
   int ncu = 0;
   double* ws = nullptr;
   int* iws = nullptr;
-  int* lists = nullptr;         // 5 lists of ws_cells ints (active x2, set-up x2, predictor)
+  double* mat = nullptr;        // per-cell matrix region (factors, Newton vectors)
+  int* lists = nullptr;         // 6 lists of ws_cells ints (active x2, set-up x2, predictor, solve)
   long ws_cells = 0;            // cells the workspace and lists hold
   unsigned int* counts = nullptr;
   unsigned long long* counters = nullptr;
@@ -74,6 +75,8 @@ void free_workspace(Reactor* r) {
   if (r->ws) (void)hipFree(r->ws);
   if (r->iws) (void)hipFree(r->iws);
   if (r->lists) (void)hipFree(r->lists);
+  if (r->mat) (void)hipFree(r->mat);
+  r->mat = nullptr;
   r->ws = nullptr;
   r->iws = nullptr;
   r->lists = nullptr;
@@ -96,8 +99,11 @@ int ensure_device_state(Reactor* r, long ncell) {
     if (hipMalloc(&r->ws, nd * sizeof(double)) != hipSuccess)
       return fail(r, MI_ENOMEM, "workspace allocation failed (" + std::to_string(nd * 8) + " bytes)");
     if (hipMalloc(&r->iws, ni * sizeof(int)) != hipSuccess) return fail(r, MI_ENOMEM, "workspace allocation failed");
-    if (hipMalloc(&r->lists, 5 * (size_t)nw * 64 * sizeof(int)) != hipSuccess)
+    if (hipMalloc(&r->lists, 6 * (size_t)nw * 64 * sizeof(int)) != hipSuccess)
       return fail(r, MI_ENOMEM, "list allocation failed");
+    const size_t nm = (size_t)nw * 64 * mat_stride(r->mech);
+    if (hipMalloc(&r->mat, nm * sizeof(double)) != hipSuccess)
+      return fail(r, MI_ENOMEM, "matrix region allocation failed (" + std::to_string(nm * 8) + " bytes)");
     r->ws_cells = nw * 64;
   }
   return MI_OK;
@@ -149,6 +155,8 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
     a.slist[i] = r->lists + (size_t)(2 + i) * r->ws_cells;
   }
   a.plist = r->lists + (size_t)4 * r->ws_cells;
+  a.nlist = r->lists + (size_t)5 * r->ws_cells;
+  a.mat = r->mat;
   a.use_pred = (r->predictor && !r->null_reactor) ? 1 : 0;
   if (a.use_pred && !a.compact) return fail(r, MI_EINVAL, "the learned predictor needs mi.schedule = queue");
   a.counts = r->counts;

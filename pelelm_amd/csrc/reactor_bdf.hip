@@ -73,8 +73,7 @@ struct Slots {
   static constexpr int ACOR = ZN + LMAX * N1;
   static constexpr int EWT = ACOR + N1;
   static constexpr int JS = EWT + N1;                 // saved Jacobian, row-major N1*N1
-  static constexpr int LU = JS + N1 * N1;             // LU factors of I - gamma J
-  static constexpr int FY = LU + N1 * N1;             // species forcing (N1-1)
+  static constexpr int FY = JS + N1 * N1;             // species forcing (N1-1)
   static constexpr int TMP = FY + N1;                 // N1 scratch vector
   static constexpr int ATOL = TMP + N1;               // N1
   static constexpr int Y = ATOL + N1;                 // evaluation point of this trip
@@ -84,8 +83,23 @@ struct Slots {
   static constexpr int LANE = SING + 1;               // integrator state words
   static constexpr int LANE_WORDS = 96;
   static constexpr int TOTAL = LANE + LANE_WORDS;
-  static constexpr int PERM = 0;                      // int slots: row i of LU is row PERM[i] of M
-  static constexpr int ITOTAL = N1;
+  static constexpr int ITOTAL = 1;                    // int slots (none in use)
+};
+
+// Per-cell matrix region, cell-major (a cell's words are contiguous, so the 32 lanes
+// that factor or solve one cell read and write whole lines): the factors of
+// M = I - gamma J (row i holds row PERM[i] of M; FP64, or FP32 packed two per word),
+// the Newton right-hand side B and solution X, refinement information and PERM.
+template <int N1>
+struct MatSlots {
+  static constexpr int LU = 0;
+  static constexpr int B = N1 * N1;
+  static constexpr int X = B + N1;
+  static constexpr int IRS = X + N1;                  // refinement sweeps of the last solve
+  static constexpr int IRF = IRS + 1;                 // 1: refinement did not converge
+  static constexpr int PERM = IRF + 1;                // N1 ints packed two per word
+  static constexpr int USED = PERM + (N1 + 1) / 2;
+  static constexpr int STRIDE = (USED + 15) / 16 * 16; // whole 128-byte lines per cell
 };
 
 struct WS {
@@ -95,6 +109,12 @@ struct WS {
 struct IWS {
   int* p;
   __device__ __forceinline__ int& operator[](int s) const { return p[(long)s * 64]; }
+};
+// one cell's matrix region
+struct CM {
+  double* p;
+  __device__ __forceinline__ double& operator[](int s) const { return p[s]; }
+  __device__ __forceinline__ int& perm(int base, int i) const { return reinterpret_cast<int*>(p + base)[i]; }
 };
 
 __device__ __forceinline__ long cell_off(const ArrView& v, int i, int j, int k) {
@@ -202,99 +222,105 @@ __device__ __noinline__ void jacobian(const double* __restrict__ y, const double
 // ---------------------------------------------------------------------------
 // dense LU of M = I - gamma J (partial pivoting, whole-row interchanges) and solve
 // ---------------------------------------------------------------------------
-// Factors of I - gamma J live in the LU slots: FP64 one per slot, or FP32 two per slot
-// (element e in half e & 1 of slot e >> 1, so a wave's loads of e and e + 1 share lines).
+// Factors of I - gamma J in the matrix region: FP64 one per word, or FP32 contiguous
+// (element e at float index e of the region).
 template <int N1, class Real>
 struct LUAcc;
 template <int N1>
 struct LUAcc<N1, double> {
-  static __device__ __forceinline__ double& at(const WS& w, int e) { return w[Slots<N1>::LU + e]; }
+  static __device__ __forceinline__ double& at(const CM& m, int e) { return m[MatSlots<N1>::LU + e]; }
 };
 template <int N1>
 struct LUAcc<N1, float> {
-  static __device__ __forceinline__ float& at(const WS& w, int e) {
-    return reinterpret_cast<float*>(&w[Slots<N1>::LU + (e >> 1)])[e & 1];
+  static __device__ __forceinline__ float& at(const CM& m, int e) {
+    return reinterpret_cast<float*>(&m[MatSlots<N1>::LU])[e];
   }
 };
 
 // one lane factors its cell's M = I - gamma J (partial pivoting, arithmetic in Real);
 // returns 0, or 1 + the first column without a nonzero pivot
 template <int N1, class Real>
-__device__ __noinline__ int lu_factor(const WS& ws, const IWS& iws, const double gamma) {
+__device__ __noinline__ int lu_factor(const WS& ws, const CM& mat, const double gamma) {
   using S = Slots<N1>;
+  using MS = MatSlots<N1>;
   using A = LUAcc<N1, Real>;
 #pragma unroll 1
   for (int i = 0; i < N1; ++i) {
-    iws[S::PERM + i] = i;
+    mat.perm(MS::PERM, i) = i;
 #pragma unroll
-    for (int j = 0; j < N1; ++j) A::at(ws, i * N1 + j) = (Real)((i == j ? 1.0 : 0.0) - gamma * ws[S::JS + i * N1 + j]);
+    for (int j = 0; j < N1; ++j) A::at(mat, i * N1 + j) = (Real)((i == j ? 1.0 : 0.0) - gamma * ws[S::JS + i * N1 + j]);
   }
   int sing = 0;
 #pragma unroll 1
   for (int k = 0; k < N1; ++k) {
     int p = k;
-    Real amax = fabs(A::at(ws, k * N1 + k));
+    Real amax = fabs(A::at(mat, k * N1 + k));
 #pragma unroll 1
     for (int i = k + 1; i < N1; ++i) {
-      const Real a = fabs(A::at(ws, i * N1 + k));
+      const Real a = fabs(A::at(mat, i * N1 + k));
       if (a > amax) { amax = a; p = i; }
     }
     if (amax == (Real)0) { if (!sing) sing = k + 1; continue; }
     if (p != k) {
-      const int t = iws[S::PERM + k];
-      iws[S::PERM + k] = iws[S::PERM + p];
-      iws[S::PERM + p] = t;
+      const int t = mat.perm(MS::PERM, k);
+      mat.perm(MS::PERM, k) = mat.perm(MS::PERM, p);
+      mat.perm(MS::PERM, p) = t;
 #pragma unroll 1
       for (int j = 0; j < N1; ++j) {
-        const Real t2 = A::at(ws, k * N1 + j);
-        A::at(ws, k * N1 + j) = A::at(ws, p * N1 + j);
-        A::at(ws, p * N1 + j) = t2;
+        const Real t2 = A::at(mat, k * N1 + j);
+        A::at(mat, k * N1 + j) = A::at(mat, p * N1 + j);
+        A::at(mat, p * N1 + j) = t2;
       }
     }
-    const Real inv = (Real)1 / A::at(ws, k * N1 + k);
+    const Real inv = (Real)1 / A::at(mat, k * N1 + k);
 #pragma unroll 1
     for (int i = k + 1; i < N1; ++i) {
-      const Real lik = A::at(ws, i * N1 + k) * inv;
-      A::at(ws, i * N1 + k) = lik;
+      const Real lik = A::at(mat, i * N1 + k) * inv;
+      A::at(mat, i * N1 + k) = lik;
       if (lik != (Real)0) {
 #pragma unroll 1
-        for (int j = k + 1; j < N1; ++j) A::at(ws, i * N1 + j) -= lik * A::at(ws, k * N1 + j);
+        for (int j = k + 1; j < N1; ++j) A::at(mat, i * N1 + j) -= lik * A::at(mat, k * N1 + j);
       }
     }
   }
   return sing;
 }
 
-// solve M x = b with b given in ws[TMP]; result in x (registers), FP64 arithmetic on
-// factors stored in Real.  P M = L U with row i of the factors holding row PERM[i] of M.
+// one lane solves M x = b (b in registers) with the factors of the matrix region, FP64
+// arithmetic on factors stored in Real.  P M = L U, row i of the factors holding row
+// PERM[i] of M.
 template <int N1, class Real = double>
-__device__ __forceinline__ void lu_solve(const WS& ws, const IWS& iws, double* __restrict__ x) {
-  using S = Slots<N1>;
+__device__ __forceinline__ void lu_solve(const CM& mat, const double* __restrict__ b, double* __restrict__ x) {
+  using MS = MatSlots<N1>;
   using A = LUAcc<N1, Real>;
 #pragma unroll
-  for (int i = 0; i < N1; ++i) x[i] = ws[S::TMP + iws[S::PERM + i]];
+  for (int i = 0; i < N1; ++i) {
+    const int pi = mat.perm(MS::PERM, i);
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) v = (j == pi) ? b[j] : v;   // register gather of b[PERM[i]]
+    x[i] = v;
+  }
 #pragma unroll
   for (int k = 0; k < N1; ++k)
 #pragma unroll
-    for (int i = k + 1; i < N1; ++i) x[i] -= (double)A::at(ws, i * N1 + k) * x[k];
+    for (int i = k + 1; i < N1; ++i) x[i] -= (double)A::at(mat, i * N1 + k) * x[k];
 #pragma unroll
   for (int k = N1 - 1; k >= 0; --k) {
     double s = x[k];
 #pragma unroll
-    for (int j = k + 1; j < N1; ++j) s -= (double)A::at(ws, k * N1 + j) * x[j];
-    x[k] = s / (double)A::at(ws, k * N1 + k);
+    for (int j = k + 1; j < N1; ++j) s -= (double)A::at(mat, k * N1 + j) * x[j];
+    x[k] = s / (double)A::at(mat, k * N1 + k);
   }
 }
 
-// ---------------------------------------------------------------------------
-// per-lane integrator state (registers) and the CVODE step logic on it
-// ---------------------------------------------------------------------------
 template <class M>
 struct Lane {
   static constexpr int N = M::NS, N1 = M::N1;
   using S = Slots<N1>;
   WS ws;
   IWS iws;
+  CM mat;
   int qmax;
   double rtol;
   long mxstep;
@@ -754,42 +780,18 @@ struct Lane {
     return true;
   }
 
-  // x = M^-1 b for b in ws[TMP] with FP32 factors of M = I - gammap J and up to
-  // ir_sweeps FP64-residual refinement sweeps (r = b - M x from the saved FP64 Jacobian,
-  // x += M^-1 r); false when the last correction is still above IR_TOL or not finite
-  __device__ bool solve_refined(double* x) {
-    double b[N1];
-#pragma unroll
-    for (int i = 0; i < N1; ++i) b[i] = ws[S::TMP + i];
-    lu_solve<N1, float>(ws, iws, x);
-    if (ir_sweeps == 0) return true;
-#pragma unroll 1
-    for (int sweep = 0; sweep < ir_sweeps; ++sweep) {
-      ++nir;
-#pragma unroll 1
-      for (int i = 0; i < N1; ++i) {
-        double r = b[i] - x[i];
-#pragma unroll
-        for (int j = 0; j < N1; ++j) r += gammap * ws[S::JS + i * N1 + j] * x[j];
-        ws[S::TMP + i] = r;
-      }
-      double d[N1];
-      lu_solve<N1, float>(ws, iws, d);
-#pragma unroll
-      for (int i = 0; i < N1; ++i) x[i] += d[i];
-      if (wrms_reg(d) <= IR_TOL) return true;
-    }
-    return false;   // also for a NaN norm
-  }
-
-  // one Newton iteration with f = f(tn, zn0 + acor); returns true when tout is reached
-  __device__ bool iterate(const double* f) {
+  // one Newton iteration: x solves M x = gamma f(tn, zn0 + acor) - rl1 zn1 - acor (the
+  // right-hand side is formed by the evaluation kernel, the solve by the solve kernel,
+  // with FP64-residual refinement of FP32 factors); returns true when tout is reached
+  __device__ bool iterate() {
+    using MS = MatSlots<N1>;
     ++nni;
-#pragma unroll
-    for (int i = 0; i < N1; ++i) ws[S::TMP + i] = gamma * f[i] - rl1 * zn(1, i) - ws[S::ACOR + i];
     double x[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) x[i] = mat[MS::X + i];
     if (lu_mode == LU_FP32_IR && !lu64) {
-      if (!solve_refined(x)) {
+      nir += (long)mat[MS::IRS];
+      if (mat[MS::IRF] != 0.0) {
         // the FP32 factors cannot deliver this solve: refactor in FP64 and restart the
         // Newton iteration from the prediction (the Jacobian is kept unless stale)
         ++nlufb;
@@ -802,8 +804,6 @@ struct Lane {
         a1 = 0.0; a2 = 0.0; teval = tn;
         return false;
       }
-    } else {
-      lu_solve<N1>(ws, iws, x);
     }
     if (gamrat != 1.0) {
       const double sc = 2.0 / (1.0 + gamrat);
@@ -888,9 +888,16 @@ __device__ __forceinline__ IWS cell_iws(int* iws, long c) {
   return IWS{iws + (c >> 6) * (long)Slots<M::N1>::ITOTAL * 64 + (c & 63)};
 }
 
+template <class M>
+__device__ __forceinline__ CM cell_mat(double* mat, long c) {
+  return CM{mat + c * (long)MatSlots<M::N1>::STRIDE};
+}
+
 // the lane state persists between the kernels of a trip as words of the workspace
 template <class M>
-__device__ __forceinline__ void load_lane(Lane<M>& st, const WS& w, const IWS& iw, const ReactArgs& a) {
+__device__ __forceinline__ void load_lane(Lane<M>& st, const ReactArgs& a, long c) {
+  const WS w = cell_ws<M>(a.ws, c);
+  const IWS iw = cell_iws<M>(a.iws, c);
   using S = Slots<M::N1>;
   constexpr int NW = (int)(sizeof(Lane<M>) / 8);
   static_assert(sizeof(Lane<M>) % 8 == 0 && NW <= S::LANE_WORDS, "lane state does not fit its slots");
@@ -900,6 +907,7 @@ __device__ __forceinline__ void load_lane(Lane<M>& st, const WS& w, const IWS& i
   __builtin_memcpy(&st, tmp, sizeof(Lane<M>));
   st.ws = w;
   st.iws = iw;
+  st.mat = cell_mat<M>(a.mat, c);
   st.qmax = a.qmax;
   st.lu_mode = a.lu_mode;
   st.ir_sweeps = a.ir_sweeps;
@@ -988,6 +996,7 @@ __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
     Lane<M> st;
     st.ws = w;
     st.iws = cell_iws<M>(a.iws, c);
+    st.mat = cell_mat<M>(a.mat, c);
     st.qmax = a.qmax;
     st.lu_mode = a.lu_mode;
     st.ir_sweeps = a.ir_sweeps;
@@ -1029,13 +1038,17 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) k_eval(const ReactArg
                                                                   const unsigned int* __restrict__ count) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
+  using MS = MatSlots<N1>;
   const long idx = (long)blockIdx.x * 64 + threadIdx.x;
   int iters = 0;
+  bool newton = false;
+  long c = -1;
   if (idx < (long)*count) {
-    const long c = list[idx];
+    c = list[idx];
     const WS w = cell_ws<M>(a.ws, c);
     // only the fields the evaluation needs are read, not the whole lane state
-    if (LANE_FIELD(M, w, phase) != PH_NEED_CELL) {
+    const int phase = LANE_FIELD(M, w, phase);
+    if (phase != PH_NEED_CELL) {
       const double a1 = LANE_FIELD(M, w, a1), a2 = LANE_FIELD(M, w, a2);
       double y[N1], f[N1], Tsol;
 #pragma unroll
@@ -1047,8 +1060,17 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) k_eval(const ReactArg
         w[S::F + i] = f[i];
       }
       w[S::TS] = Tsol;
+      if (phase == PH_NEWTON) {
+        // the Newton system's right-hand side gamma f - rl1 zn1 - acor
+        newton = true;
+        const double gamma = LANE_FIELD(M, w, gamma), rl1 = LANE_FIELD(M, w, rl1);
+        const CM mat = cell_mat<M>(a.mat, c);
+#pragma unroll
+        for (int i = 0; i < N1; ++i) mat[MS::B + i] = gamma * f[i] - rl1 * w[S::ZN + N1 + i] - w[S::ACOR + i];
+      }
     }
   }
+  wave_append(newton, (int)c, a.nlist, &a.counts[6]);
   if (a.counters) wave_add(&a.counters[CNT_HY2T], (unsigned long long)iters);
 }
 
@@ -1063,7 +1085,7 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_SETUP_WAVES) k_setu
   const long c = list[idx];
   const WS w = cell_ws<M>(a.ws, c);
   Lane<M> st;
-  load_lane(st, w, cell_iws<M>(a.iws, c), a);
+  load_lane(st, a, c);
   if (!st.needs_setup() || st.setup_pending) return;
   double y[N1], f[N1];
 #pragma unroll
@@ -1145,7 +1167,7 @@ __global__ void __launch_bounds__(MLP_THREADS) k_predict(const ReactArgs a, cons
     const long c = list[base + t];
     const WS wc = cell_ws<M>(a.ws, c);
     Lane<M> st;
-    load_lane(st, wc, cell_iws<M>(a.iws, c), a);
+    load_lane(st, a, c);
     if (st.pred_state == 1) {
       double rho = 0.0;
       for (int k = 0; k < N; ++k) rho += wc[S::TMP + k];
@@ -1184,11 +1206,11 @@ __global__ void __launch_bounds__(64) k_lu_lane(const ReactArgs a, const int* __
   const long c = list[idx];
   const WS w = cell_ws<M>(a.ws, c);
   Lane<M> st;
-  load_lane(st, w, cell_iws<M>(a.iws, c), a);
+  load_lane(st, a, c);
   if (!st.setup_pending) return;
   const bool fp32 = (st.lu_mode == LU_FP32_IR) && !st.lu64;
-  w[S::SING] = (double)(fp32 ? lu_factor<M::N1, float>(w, st.iws, st.gamma)
-                             : lu_factor<M::N1, double>(w, st.iws, st.gamma));
+  w[S::SING] = (double)(fp32 ? lu_factor<M::N1, float>(w, st.mat, st.gamma)
+                             : lu_factor<M::N1, double>(w, st.mat, st.gamma));
 }
 
 
@@ -1278,12 +1300,161 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
   }
   if (row) {
     const WS w = cell_ws<M>(a.ws, c);
-    const IWS iw = cell_iws<M>(a.iws, c);
+    const CM mat = cell_mat<M>(a.mat, c);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) A::at(w, my_step * N1 + j) = m[j];
-    iw[S::PERM + my_step] = r;
+    for (int j = 0; j < N1; ++j) A::at(mat, my_step * N1 + j) = m[j];
+    mat.perm(MatSlots<N1>::PERM, my_step) = r;
     if (r == 0) w[S::SING] = (double)sing;
   }
+}
+
+// true when the cell's Newton iteration proceeds this trip (not a singular set-up)
+template <class M>
+__device__ __forceinline__ bool solve_due(const WS& w) {
+  using S = Slots<M::N1>;
+  return !(LANE_FIELD(M, w, setup_pending) != 0 && w[S::SING] != 0.0);
+}
+
+// Newton solves, one lane per cell (the form that runs when the kernels execute
+// serially on a CPU): x = M^-1 B with FP64 factors, or with FP32 factors and up to
+// ir_sweeps refinement sweeps r = B - M x (M = I - gammap J from the FP64 Jacobian),
+// x += M^-1 r, stopping once the correction's WRMS norm is at most IR_TOL.
+template <class M>
+__global__ void __launch_bounds__(64) k_solve_lane(const ReactArgs a, const int* __restrict__ list,
+                                                   const unsigned int* __restrict__ count) {
+  constexpr int N1 = M::N1;
+  using S = Slots<N1>;
+  using MS = MatSlots<N1>;
+  const long idx = (long)blockIdx.x * 64 + threadIdx.x;
+  if (idx >= (long)*count) return;
+  const long c = list[idx];
+  const WS w = cell_ws<M>(a.ws, c);
+  if (!solve_due<M>(w)) return;
+  const CM mat = cell_mat<M>(a.mat, c);
+  double b[N1], x[N1];
+#pragma unroll
+  for (int i = 0; i < N1; ++i) b[i] = mat[MS::B + i];
+  const bool fp32 = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
+  if (!fp32) {
+    lu_solve<N1, double>(mat, b, x);
+  } else {
+    lu_solve<N1, float>(mat, b, x);
+    const double gp = LANE_FIELD(M, w, gammap);
+    int sweeps = 0;
+    bool ok = (a.ir_sweeps == 0);
+#pragma unroll 1
+    for (; sweeps < a.ir_sweeps && !ok;) {
+      ++sweeps;
+      double r[N1], d[N1];
+#pragma unroll 1
+      for (int i = 0; i < N1; ++i) {
+        double v = b[i] - x[i];
+#pragma unroll
+        for (int j = 0; j < N1; ++j) v += gp * w[S::JS + i * N1 + j] * x[j];
+        r[i] = v;
+      }
+      lu_solve<N1, float>(mat, r, d);
+      double nrm = 0.0;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        x[i] += d[i];
+        const double q = d[i] * w[S::EWT + i];
+        nrm += q * q;
+      }
+      ok = (sqrt(nrm / N1) <= IR_TOL);   // false for a NaN norm
+    }
+    mat[MS::IRS] = (double)sweeps;
+    mat[MS::IRF] = ok ? 0.0 : 1.0;
+  }
+#pragma unroll
+  for (int i = 0; i < N1; ++i) mat[MS::X + i] = x[i];
+}
+
+// The same solves with 32 lanes per cell (row r of the factors in lane r), two cells
+// per wave: forward and back substitution broadcast each finished component with a
+// 32-lane shuffle; a refinement residual gathers x the same way.  A launch handles the
+// cells whose factor precision is Real.  No barriers: the two halves of a wave may
+// take different numbers of refinement sweeps.
+template <class M, class Real>
+__global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int* __restrict__ list,
+                                                   const unsigned int* __restrict__ count) {
+  constexpr int N1 = M::N1;
+  static_assert(N1 <= 32, "one row per lane of a 32-lane half-wave");
+  using S = Slots<N1>;
+  using MS = MatSlots<N1>;
+  using A = LUAcc<N1, Real>;
+  constexpr bool IS32 = sizeof(Real) == 4;
+  const int half = threadIdx.x >> 5, r = threadIdx.x & 31;
+  const unsigned per_xcd = gridDim.x / 8;
+  const long blk = (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  const long idx = blk * 2 + half;
+  bool work = false;
+  long c = 0;
+  if (idx < (long)*count) {
+    c = list[idx];
+    const WS w = cell_ws<M>(a.ws, c);
+    const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
+    work = solve_due<M>(w) && (fp32_cell == IS32);
+  }
+  const bool row = work && r < N1;
+  const CM mat = cell_mat<M>(a.mat, c);
+  Real lrow[N1];
+  int pr = 0;
+  if (row) {
+#pragma unroll
+    for (int j = 0; j < N1; ++j) lrow[j] = A::at(mat, r * N1 + j);
+    pr = mat.perm(MS::PERM, r);
+  } else {
+#pragma unroll
+    for (int j = 0; j < N1; ++j) lrow[j] = (Real)0;
+  }
+  // x = U^-1 L^-1 (P v) where lane r holds v[PERM[r]] on entry
+  auto subst = [&](double xv) {
+#pragma unroll
+    for (int k = 0; k < N1 - 1; ++k) {
+      const double xk = __shfl(xv, k, 32);
+      if (r > k) xv -= (double)lrow[k] * xk;
+    }
+#pragma unroll
+    for (int k = N1 - 1; k >= 0; --k) {
+      if (r == k) xv /= (double)lrow[k];
+      const double xk = __shfl(xv, k, 32);
+      if (r < k) xv -= (double)lrow[k] * xk;
+    }
+    return row ? xv : 0.0;
+  };
+  double x = subst(row ? mat[MS::B + pr] : 0.0);
+  if (IS32) {
+    // refinement of the FP32 solve against the FP64 matrix (uniform per half-wave)
+    const WS w = cell_ws<M>(a.ws, c);
+    const double gp = work ? LANE_FIELD(M, w, gammap) : 0.0;
+    const double bi = row ? mat[MS::B + r] : 0.0;
+    const double ewt = row ? w[S::EWT + r] : 0.0;
+    int sweeps = 0;
+    bool ok = !work || (a.ir_sweeps == 0);
+#pragma unroll 1
+    for (; sweeps < a.ir_sweeps && !ok;) {
+      ++sweeps;
+      double res = bi - x;
+#pragma unroll
+      for (int j = 0; j < N1; ++j) {
+        const double xj = __shfl(x, j, 32);
+        if (row) res += gp * w[S::JS + r * N1 + j] * xj;
+      }
+      const double d = subst(__shfl(res, pr, 32));
+      x += d;
+      double q = d * ewt;
+      q *= q;
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) q += __shfl_xor(q, off, 32);
+      ok = (sqrt(q / N1) <= IR_TOL);
+    }
+    if (work && r == 0) {
+      mat[MS::IRS] = (double)sweeps;
+      mat[MS::IRF] = ok ? 0.0 : 1.0;
+    }
+  }
+  if (row) mat[MS::X + r] = x;
 }
 
 // advance every active cell's state machine by this trip's evaluation
@@ -1306,7 +1477,7 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_co
   if (c >= 0 && LANE_FIELD(M, w0, phase) != PH_NEED_CELL) {
     const WS w = w0;
     Lane<M> st;
-    load_lane(st, w, cell_iws<M>(a.iws, c), a);
+    load_lane(st, a, c);
     {
       ++st.nfe;
       double f[N1];
@@ -1326,7 +1497,7 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_co
             st.after_hin(f);
             break;
           case PH_NEWTON:
-            reached = st.iterate(f);
+            reached = st.iterate();
             break;
           case PH_RESTART_F:
             st.restart_f(f);
@@ -1412,7 +1583,7 @@ __global__ void __launch_bounds__(64) k_abandon(const ReactArgs a, const int* __
     const long c = list[idx];
     const WS w = cell_ws<M>(a.ws, c);
     Lane<M> st;
-    load_lane(st, w, cell_iws<M>(a.iws, c), a);
+    load_lane(st, a, c);
     if (st.phase != PH_NEED_CELL) {
       int ci, cj, ck;
       cell_ijk(a, c, ci, cj, ck);
@@ -1524,9 +1695,27 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
   return hipGetLastError();
 }
 
+template <class M>
+hipError_t launch_solve(const ReactArgs& a, long n, hipStream_t stream) {
+  if (a.lu_lane) {
+    hipLaunchKernelGGL(k_solve_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, a.nlist,
+                       &a.counts[6]);
+    return hipGetLastError();
+  }
+  const dim3 grid((unsigned)(((n + 1) / 2 + 7) / 8 * 8));
+  if (a.lu_mode == LU_FP32_IR) {
+    hipLaunchKernelGGL((k_solve_coop<M, float>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_solve_coop<M, double>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
+  return hipGetLastError();
+}
+
 // counts[0..1]: active-cell lists of the current / next trip, counts[2..3]: their set-up
 // lists; without compaction counts[4] is the (fixed) list length and counts[5] the
-// number of cells still integrating; counts[7]: cells awaiting a learned prediction.
+// number of cells still integrating; counts[6]: cells due a Newton solve this trip;
+// counts[7]: cells awaiting a learned prediction.
 // Per-kernel timing: an event pair per launch, recorded asynchronously and read back
 // once the call has finished, so profiling adds no synchronisation inside the trips.
 struct Timer {
@@ -1614,6 +1803,8 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
       const int nxt = 1 - cur;
       hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, nxt, 2 + nxt);
       PELE_TRY(hipGetLastError());
+      hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, 6, 6);
+      PELE_TRY(hipGetLastError());
       const unsigned ga = (active + 63) / 64;
       if (a.use_pred && h_counts[7] > 0) {
         const unsigned np = h_counts[7];
@@ -1638,6 +1829,9 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
         PELE_TRY(tm.end(KK_LU, nsetup));
       }
       PELE_TRY(tm.begin());
+      PELE_TRY(launch_solve<M>(a, active, stream));
+      PELE_TRY(tm.end(KK_SOLVE, active));
+      PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_consume<M>, dim3(ga), dim3(64), 0, stream, a, a.list[cur], &a.counts[cur], a.list[nxt],
                          &a.counts[nxt], a.slist[nxt], &a.counts[2 + nxt]);
       PELE_TRY(hipGetLastError());
@@ -1648,6 +1842,8 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
       const unsigned int nsetup = h_counts[2 + cur];
       const int nxt = 1 - cur;
       hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, 5, 2 + nxt);
+      PELE_TRY(hipGetLastError());
+      hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, stream, a.counts, 6, 6);
       PELE_TRY(hipGetLastError());
       PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_eval<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4]);
@@ -1663,6 +1859,9 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
         PELE_TRY(launch_lu<M>(a, a.slist[cur], &a.counts[2 + cur], nsetup, stream));
         PELE_TRY(tm.end(KK_LU, nsetup));
       }
+      PELE_TRY(tm.begin());
+      PELE_TRY(launch_solve<M>(a, h_counts[5], stream));
+      PELE_TRY(tm.end(KK_SOLVE, h_counts[5]));
       PELE_TRY(tm.begin());
       hipLaunchKernelGGL(k_consume<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4], a.list[0],
                          &a.counts[5], a.slist[nxt], &a.counts[2 + nxt]);
@@ -1727,6 +1926,14 @@ long iws_slots(int mech) {
   switch (mech) {
     case MECH_LIDRYER: return Slots<mech_LiDryer::N1>::ITOTAL;
     case MECH_DRM19: return Slots<mech_drm19::N1>::ITOTAL;
+    default: return 0;
+  }
+}
+
+long mat_stride(int mech) {
+  switch (mech) {
+    case MECH_LIDRYER: return MatSlots<mech_LiDryer::N1>::STRIDE;
+    case MECH_DRM19: return MatSlots<mech_drm19::N1>::STRIDE;
     default: return 0;
   }
 }

@@ -47,11 +47,14 @@ struct ReactArgs {
   // ws[((c / 64) * nslot + s) * 64 + c % 64]
   double* ws;
   int* iws;
+  // per-cell matrix region, cell-major: cell c's words at mat[c * mat_stride ...]
+  double* mat;
   // trip pipeline: active-cell lists (current / next), set-up lists and their counts
   int compact;                           // 1: finished cells leave the lists
   int* list[2];
   int* slist[2];
   int* plist;                            // cells whose next attempt awaits the learned prediction
+  int* nlist;                            // cells due a Newton solve this trip
   int use_pred;                          // 1: the learned predictor supplies each Newton attempt's
                                          // initial iterate (compacted schedule only)
   unsigned int* counts;                  // 8 counters, see run_react
@@ -73,6 +76,7 @@ const char* mech_name(int mech);
 int mech_lookup(const char* name);
 long ws_slots(int mech);        // doubles per lane
 long iws_slots(int mech);       // ints per lane
+long mat_stride(int mech);      // doubles per cell in the matrix region
 
 // Integrate every cell described by `a`: trips of bounded kernels on `stream`, driven
 // from the host until no cell is active or max_trips trips have run (cells still
@@ -80,7 +84,7 @@ long iws_slots(int mech);       // ints per lane
 // Optional per-kernel timing: when `prof` is non-null every launch of the trip
 // kernels is bracketed by events and the per-kind device time (ms), launch count and
 // cells processed are accumulated into prof[kind].
-enum KernelKind : int { KK_EVAL = 0, KK_SETUP, KK_CONSUME, KK_LU, KK_PRED, NKK };
+enum KernelKind : int { KK_EVAL = 0, KK_SETUP, KK_CONSUME, KK_LU, KK_PRED, KK_SOLVE, NKK };
 struct KernelProf {
   double ms[NKK];
   long launches[NKK];

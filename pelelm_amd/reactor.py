@@ -39,8 +39,8 @@ class MiStats(C.Structure):
                 ("nfail", C.c_long), ("max_steps_cell", C.c_long), ("nlu_refine", C.c_long),
                 ("nlu_fallback", C.c_long), ("npred_used", C.c_long), ("npred_fallback", C.c_long),
                 ("ms_kernel", C.c_double), ("ntrips", C.c_long), ("nhy2t_iters", C.c_long),
-                ("kernel_ms", C.c_double * 5),
-                ("kernel_launches", C.c_long * 5), ("kernel_cells", C.c_long * 5)]
+                ("kernel_ms", C.c_double * 6),
+                ("kernel_launches", C.c_long * 6), ("kernel_cells", C.c_long * 6)]
 
 
 # Every exported entry point of include/pelelm_chem.h.

@@ -32,9 +32,12 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   const long nw = (ncell + 63) / 64;
   double* ws = new double[(size_t)nw * ws_slots(m) * 64];
   int* iws = new int[(size_t)nw * iws_slots(m) * 64];
-  int* lists = new int[4 * (size_t)nw * 64];
+  int* lists = new int[6 * (size_t)nw * 64];
+  double* mat = new double[(size_t)nw * 64 * mat_stride(m)];
   unsigned int counts[8] = {0};
-  a.ws = ws; a.iws = iws;
+  a.ws = ws; a.iws = iws; a.mat = mat;
+  a.plist = lists + (size_t)4 * nw * 64;
+  a.nlist = lists + (size_t)5 * nw * 64;
   for (int i = 0; i < 2; ++i) {
     a.list[i] = lists + (size_t)i * nw * 64;
     a.slist[i] = lists + (size_t)(2 + i) * nw * 64;
@@ -49,6 +52,7 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   delete[] ws;
   delete[] iws;
   delete[] lists;
+  delete[] mat;
   return (int)e;
 }
 

@@ -47,14 +47,14 @@ __global__ void k_rhs(const double* y, const double* t, const double* rE0, const
 // solve for b; every lane carries a live vector through the branch and writes it back.
 template <class M>
 __global__ void k_jac_lu(const double* y, const double* T, const double* fT, const double* gamma,
-                         const double* b, double* J, double* x, double* keep_out, int* sing, double* ws, int* iws,
+                         const double* b, double* J, double* x, double* keep_out, int* sing, double* ws, double* mats,
                          long ncell) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
   const long c = (long)blockIdx.x * 64 + threadIdx.x;
   if (c >= ncell) return;
   WS w{ws + (c >> 6) * (long)S::TOTAL * 64 + (c & 63)};
-  IWS iw{iws + (c >> 6) * (long)S::ITOTAL * 64 + (c & 63)};
+  const CM mat = cell_mat<M>(mats, c);
   double keep[N1];
   for (int i = 0; i < N1; ++i) keep[i] = y[c * N1 + i] * 3.0 + (double)c;
   // a private array indexed with a run-time value (so it lives in scratch) and a full
@@ -74,10 +74,10 @@ __global__ void k_jac_lu(const double* y, const double* T, const double* fT, con
     double yy[N1];
     for (int i = 0; i < N1; ++i) yy[i] = y[c * N1 + i];
     jacobian<M>(yy, T[c], fT[c], w);
-    sg = lu_factor<N1, double>(w, iw, gamma[c]);
-    for (int i = 0; i < N1; ++i) w[S::TMP + i] = b[c * N1 + i];
-    double xx[N1];
-    lu_solve<N1>(w, iw, xx);
+    sg = lu_factor<N1, double>(w, mat, gamma[c]);
+    double bb[N1], xx[N1];
+    for (int i = 0; i < N1; ++i) bb[i] = b[c * N1 + i];
+    lu_solve<N1>(mat, bb, xx);
     for (int i = 0; i < N1; ++i) x[c * N1 + i] = xx[i];
     for (int e = 0; e < N1 * N1; ++e) J[c * N1 * N1 + e] = w[S::JS + e];
   }
@@ -182,7 +182,7 @@ int unit_jac_lu_t(long n, const double* y, const double* T, const double* fT, co
   double* dk = dev_zero<double>(n * N1 + 4 * n);
   int* ds = dev_zero<int>(n);
   double* ws = dev_zero<double>(nw * Slots<N1>::TOTAL * 64);
-  int* iws = dev_zero<int>(nw * Slots<N1>::ITOTAL * 64);
+  double* iws = dev_zero<double>(nw * 64 * MatSlots<N1>::STRIDE);
   hipLaunchKernelGGL(k_jac_lu<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, dy, dT, dfT, dg, db, dJ, dx, dk, ds, ws,
                      iws, n);
   UNIT_SYNC();
@@ -277,92 +277,134 @@ int unit_stepctl_t(long n, const double* scal, const double* zn, const double* a
   return 0;
 }
 
-// LU of I - gamma J by the reactor's LU kernels (cooperative or one lane per cell) on
-// given matrices.  Cells with pend[c] == 0 have no set-up pending and must be left as
-// they were (LU slots 7.0, permutation -1, singularity word -3).
+// LU of I - gamma J and the Newton solve by the reactor's kernels (cooperative or one
+// lane per cell, FP64 or FP32 factors with FP64-residual refinement) on given
+// matrices.  Cells with pend[c] == 0 have no set-up pending and must be left as they
+// were (factor words 7.0, permutation -1, singularity word -3).  The solve runs on the
+// pending, non-singular cells with right-hand side b and error weights ewt.
 template <class M>
-__global__ void k_lu_prep(const double* J, const double* gamma, const int* pend, double* ws, int* iws, long ncell) {
+__global__ void k_lu_prep(const double* J, const double* gamma, const int* pend, const double* b, const double* ewt,
+                          double* ws, double* mats, long ncell) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
+  using MS = MatSlots<N1>;
   const long c = (long)blockIdx.x * 64 + threadIdx.x;
   if (c >= ncell) return;
   const WS w = cell_ws<M>(ws, c);
-  const IWS iw = cell_iws<M>(iws, c);
+  const CM mat = cell_mat<M>(mats, c);
   Lane<M> st;
   __builtin_memset(&st, 0, sizeof(st));
   st.gamma = gamma[c];
+  st.gammap = gamma[c];
   st.setup_pending = pend[c];
+  st.phase = PH_NEWTON;
   store_lane(st, w);
   for (int e = 0; e < N1 * N1; ++e) {
     w[S::JS + e] = J[c * N1 * N1 + e];
-    w[S::LU + e] = 7.0;
+    mat[MS::LU + e] = 7.0;
   }
-  for (int i = 0; i < N1; ++i) iw[S::PERM + i] = -1;
+  for (int i = 0; i < N1; ++i) {
+    mat.perm(MS::PERM, i) = -1;
+    mat[MS::B + i] = b[c * N1 + i];
+    mat[MS::X + i] = 0.0;
+    w[S::EWT + i] = ewt[c * N1 + i];
+  }
+  mat[MS::IRS] = -1.0;
+  mat[MS::IRF] = -1.0;
   w[S::SING] = -3.0;
 }
 
 template <class M, class Real>
-__global__ void k_lu_read(const double* b, const int* pend, double* x, double* sing, double* lu, int* perm, double* ws,
-                          int* iws, long ncell) {
+__global__ void k_lu_read(double* x, double* sing, double* lu, int* perm, double* ir, double* ws, double* mats,
+                          long ncell) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
+  using MS = MatSlots<N1>;
   const long c = (long)blockIdx.x * 64 + threadIdx.x;
   if (c >= ncell) return;
   const WS w = cell_ws<M>(ws, c);
-  const IWS iw = cell_iws<M>(iws, c);
+  const CM mat = cell_mat<M>(mats, c);
   sing[c] = w[S::SING];
-  const bool untouched = (w[S::LU] == 7.0);   // idle cells keep the marker in every slot
+  const bool untouched = (w[S::SING] == -3.0);
   for (int e = 0; e < N1 * N1; ++e)
-    lu[c * N1 * N1 + e] = untouched ? w[S::LU + e] : (double)LUAcc<N1, Real>::at(w, e);
-  for (int i = 0; i < N1; ++i) perm[c * N1 + i] = iw[S::PERM + i];
-  if (pend[c] && w[S::SING] == 0.0) {
-    for (int i = 0; i < N1; ++i) w[S::TMP + i] = b[c * N1 + i];
-    double xx[N1];
-    lu_solve<N1, Real>(w, iw, xx);
-    for (int i = 0; i < N1; ++i) x[c * N1 + i] = xx[i];
+    lu[c * N1 * N1 + e] = untouched ? mat[MS::LU + e] : (double)LUAcc<N1, Real>::at(mat, e);
+  for (int i = 0; i < N1; ++i) {
+    perm[c * N1 + i] = mat.perm(MS::PERM, i);
+    x[c * N1 + i] = mat[MS::X + i];
   }
+  ir[2 * c] = mat[MS::IRS];
+  ir[2 * c + 1] = mat[MS::IRF];
 }
 
 template <class M>
-int unit_lu_t(int coop, int fp32, long n, const double* J, const double* gamma, const int* pend, const double* b, double* x,
-              double* sing, double* lu, int* perm) {
+int unit_lu_t(int coop, int fp32, int sweeps, long n, const double* J, const double* gamma, const int* pend,
+              const double* b, const double* ewt, double* x, double* sing, double* lu, int* perm, double* ir) {
   constexpr int N1 = M::N1;
 #ifdef UNIT_HOST
-  if (coop) return -2;   // lanes of the cooperative kernel cannot run one at a time
+  if (coop) return -2;   // lanes of the cooperative kernels cannot run one at a time
 #endif
   const long nw = (n + 63) / 64;
   double *dJ = dev_copy(J, n * N1 * N1), *dg = dev_copy(gamma, n), *db = dev_copy(b, n * N1);
+  double* de = dev_copy(ewt, n * N1);
   int* dp = dev_copy(pend, n);
   double *dx = dev_zero<double>(n * N1), *ds = dev_zero<double>(n), *dlu = dev_zero<double>(n * N1 * N1);
+  double* dir = dev_zero<double>(2 * n);
   int* dperm = dev_zero<int>(n * N1);
   double* ws = dev_zero<double>(nw * Slots<N1>::TOTAL * 64);
   int* iws = dev_zero<int>(nw * Slots<N1>::ITOTAL * 64);
+  double* mats = dev_zero<double>(nw * 64 * MatSlots<N1>::STRIDE);
   std::vector<int> ident(n);
   for (long i = 0; i < n; ++i) ident[i] = (int)i;
   int* dlist = dev_copy(ident.data(), n);
-  unsigned int hn = (unsigned int)n;
-  unsigned int* dcount = dev_copy(&hn, 1);
+  unsigned int hcounts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned int* dcount = dev_copy(hcounts, 8);
   ReactArgs a;
   __builtin_memset(&a, 0, sizeof(a));
   a.ws = ws;
   a.iws = iws;
+  a.mat = mats;
+  a.counts = dcount;
   a.lu_mode = fp32 ? LU_FP32_IR : LU_FP64;
+  a.ir_sweeps = sweeps;
   a.lu_lane = coop ? 0 : 1;
-  hipLaunchKernelGGL(k_lu_prep<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, dJ, dg, dp, ws, iws, n);
-  (void)launch_lu<M>(a, dlist, dcount, n, nullptr);
+  hipLaunchKernelGGL(k_lu_prep<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, dJ, dg, dp, db, de, ws, mats, n);
+  hcounts[0] = (unsigned)n;
+  UNIT_H2D(dcount, hcounts, sizeof(hcounts));
+  (void)launch_lu<M>(a, dlist, &dcount[0], n, nullptr);
+  UNIT_SYNC();
+  // solve list: pending cells whose factorisation is regular (solve_due skips the rest)
+  std::vector<double> hs(n);
   if (fp32)
-    hipLaunchKernelGGL((k_lu_read<M, float>), dim3((unsigned)nw), dim3(64), 0, nullptr, db, dp, dx, ds, dlu, dperm, ws,
-                       iws, n);
+    hipLaunchKernelGGL((k_lu_read<M, float>), dim3((unsigned)nw), dim3(64), 0, nullptr, dx, ds, dlu, dperm, dir, ws,
+                       mats, n);
   else
-    hipLaunchKernelGGL((k_lu_read<M, double>), dim3((unsigned)nw), dim3(64), 0, nullptr, db, dp, dx, ds, dlu, dperm, ws,
-                       iws, n);
+    hipLaunchKernelGGL((k_lu_read<M, double>), dim3((unsigned)nw), dim3(64), 0, nullptr, dx, ds, dlu, dperm, dir, ws,
+                       mats, n);
+  UNIT_SYNC();
+  UNIT_D2H(hs.data(), ds, n * sizeof(double));
+  std::vector<int> sl;
+  for (long i = 0; i < n; ++i)
+    if (pend[i] && hs[i] == 0.0) sl.push_back((int)i);
+  int* dsl = dev_copy(sl.empty() ? ident.data() : sl.data(), sl.empty() ? 1 : (long)sl.size());
+  hcounts[6] = (unsigned)sl.size();
+  UNIT_H2D(dcount, hcounts, sizeof(hcounts));
+  a.nlist = dsl;
+  if (!sl.empty()) (void)launch_solve<M>(a, (long)sl.size(), nullptr);
+  if (fp32)
+    hipLaunchKernelGGL((k_lu_read<M, float>), dim3((unsigned)nw), dim3(64), 0, nullptr, dx, ds, dlu, dperm, dir, ws,
+                       mats, n);
+  else
+    hipLaunchKernelGGL((k_lu_read<M, double>), dim3((unsigned)nw), dim3(64), 0, nullptr, dx, ds, dlu, dperm, dir, ws,
+                       mats, n);
   UNIT_SYNC();
   UNIT_D2H(x, dx, n * N1 * sizeof(double));
   UNIT_D2H(sing, ds, n * sizeof(double));
   UNIT_D2H(lu, dlu, n * N1 * N1 * sizeof(double));
   UNIT_D2H(perm, dperm, n * N1 * sizeof(int));
-  for (void* p : {(void*)dJ, (void*)dg, (void*)db, (void*)dp, (void*)dx, (void*)ds, (void*)dlu, (void*)dperm,
-                  (void*)ws, (void*)iws, (void*)dlist, (void*)dcount})
+  UNIT_D2H(ir, dir, 2 * n * sizeof(double));
+  for (void* p : {(void*)dJ, (void*)dg, (void*)db, (void*)de, (void*)dp, (void*)dx, (void*)ds, (void*)dlu,
+                  (void*)dir, (void*)dperm, (void*)ws, (void*)iws, (void*)mats, (void*)dlist, (void*)dcount,
+                  (void*)dsl})
     UNIT_DEV_FREE(p);
   return 0;
 }
@@ -371,11 +413,13 @@ int unit_lu_t(int coop, int fp32, long n, const double* J, const double* gamma, 
 
 using namespace pelelm;
 
-extern "C" int unit_lu(const char* mech, int coop, int fp32, long n, const double* J, const double* gamma,
-                       const int* pend, const double* b, double* x, double* sing, double* lu, int* perm) {
+extern "C" int unit_lu(const char* mech, int coop, int fp32, int sweeps, long n, const double* J, const double* gamma,
+                       const int* pend, const double* b, const double* ewt, double* x, double* sing, double* lu,
+                       int* perm, double* ir) {
   const int m = mech_lookup(mech);
-  if (m == MECH_DRM19) return unit_lu_t<mech_drm19>(coop, fp32, n, J, gamma, pend, b, x, sing, lu, perm);
-  if (m == MECH_LIDRYER) return unit_lu_t<mech_LiDryer>(coop, fp32, n, J, gamma, pend, b, x, sing, lu, perm);
+  if (m == MECH_DRM19) return unit_lu_t<mech_drm19>(coop, fp32, sweeps, n, J, gamma, pend, b, ewt, x, sing, lu, perm, ir);
+  if (m == MECH_LIDRYER)
+    return unit_lu_t<mech_LiDryer>(coop, fp32, sweeps, n, J, gamma, pend, b, ewt, x, sing, lu, perm, ir);
   return -1;
 }
 

@@ -141,16 +141,23 @@ def test_jac_lu_gpu_matches_cpu_in_divergent_lanes(mech, wl):
     assert np.max(np.abs(xg[active] - xc[active]) / xs) < 1e-10
 
 
-def run_lu(L, mech, coop, J, gamma, pend, b, fp32=0):
+def run_lu(L, mech, coop, J, gamma, pend, b, fp32=0, sweeps=3, ewt=None):
+    """LU of I - gamma J then the Newton solve for b (pending, regular cells) by the
+    reactor's kernels; returns x, singularity word, factors, permutation and the
+    refinement (sweeps, failed) pair of every cell."""
     n, N1 = b.shape
+    if ewt is None:
+        ewt = np.ones((n, N1))
+    ewt = np.ascontiguousarray(ewt)
     x = np.zeros((n, N1))
     sing = np.zeros(n)
     lu = np.zeros((n, N1, N1))
     perm = np.zeros((n, N1), dtype=np.int32)
-    rc = L.unit_lu(mech.encode(), coop, fp32, C.c_long(n), _p(J), _p(gamma), _p(pend), _p(b), _p(x), _p(sing),
-                   _p(lu), _p(perm))
+    ir = np.zeros((n, 2))
+    rc = L.unit_lu(mech.encode(), coop, fp32, sweeps, C.c_long(n), _p(J), _p(gamma), _p(pend), _p(b), _p(ewt),
+                   _p(x), _p(sing), _p(lu), _p(perm), _p(ir))
     assert rc == 0
-    return x, sing, lu, perm
+    return x, sing, lu, perm, ir
 
 
 def _lu_case(mech, wl):
@@ -171,36 +178,46 @@ def _lu_case(mech, wl):
 @pytest.mark.gpu
 @pytest.mark.parametrize("mech,wl", CASES)
 def test_cooperative_fp32_lu_matches_lane_fp32_lu(mech, wl):
-    """FP32 factors (two per workspace slot) from the cooperative kernel against the
-    one-lane FP32 factorisation, on the GPU and on the CPU twin; the FP32 solve's error
-    against numpy is what the refinement sweeps then remove."""
+    """FP32 factors from the cooperative kernel against the one-lane FP32 factorisation,
+    on the GPU and on the CPU twin (each reproduces P M to FP32 accuracy), and the
+    Newton solve with them: unrefined it carries the FP32 error, with up to 3 FP64-
+    residual refinement sweeps it reaches the FP64 solution, in both solve kernels."""
     G, Cl = _load("libunit_gpu.so"), _load("libunit_cpu.so")
     N1, n, J, gamma, pend, b = _lu_case(mech, wl)
-    xc, sc, luc, pc = run_lu(G, mech, 1, J, gamma, pend, b, fp32=1)
-    xl, sl, lul, pl = run_lu(G, mech, 0, J, gamma, pend, b, fp32=1)
-    xh, sh, luh, ph = run_lu(Cl, mech, 0, J, gamma, pend, b, fp32=1)
+    M = np.eye(N1)[None] - gamma[:, None, None] * J
+    xe = np.linalg.solve(M, b[..., None])[..., 0]
+    xs_all = np.abs(xe).max(axis=1, keepdims=True)
+    ewt = 1.0 / (1e-6 * np.abs(xe) + 1e-10 * xs_all)
+    xc, sc, luc, pc, irc = run_lu(G, mech, 1, J, gamma, pend, b, fp32=1, sweeps=3, ewt=ewt)
+    xl, sl, lul, pl, irl = run_lu(G, mech, 0, J, gamma, pend, b, fp32=1, sweeps=3, ewt=ewt)
+    xh, sh, luh, ph, irh = run_lu(Cl, mech, 0, J, gamma, pend, b, fp32=1, sweeps=3, ewt=ewt)
+    x0, _, _, _, ir0 = run_lu(G, mech, 1, J, gamma, pend, b, fp32=1, sweeps=0, ewt=ewt)
     idle = pend == 0
     assert np.all(sc[idle] == -3.0) and np.all(luc[idle] == 7.0) and np.all(pc[idle] == -1)
     ok = pend == 1
     assert np.all(sc[ok] == 0) and np.all(sl[ok] == 0) and np.all(sh[ok] == 0)
-    # same pivots wherever the FP32 column maxima are not tied
     assert np.mean(np.all(pc[ok] == pl[ok], axis=1)) > 0.99
     assert np.mean(np.all(pl[ok] == ph[ok], axis=1)) > 0.99
     same = np.all(pc == pl, axis=1) & ok
     scale = np.abs(lul[same]).max(axis=2, keepdims=True)
     assert np.max(np.abs(luc[same] - lul[same]) / scale) < 1e-5
-    M = np.eye(N1)[None] - gamma[ok, None, None] * J[ok]
+    Mo = M[ok]
     # each factorisation reproduces P M to FP32 accuracy (the GPU contracts to FMA, the
     # CPU twin does not, so their factors differ where elimination cancels)
     for lu, pm in ((luc, pc), (lul, pl), (luh, ph)):
         L = np.tril(lu[ok], -1) + np.eye(N1)[None]
         U = np.triu(lu[ok])
-        PM = np.take_along_axis(M, pm[ok][:, :, None].astype(np.int64), axis=1)
-        res = np.abs(PM - L @ U).max(axis=(1, 2)) / np.abs(M).max(axis=(1, 2))
+        PM = np.take_along_axis(Mo, pm[ok][:, :, None].astype(np.int64), axis=1)
+        res = np.abs(PM - L @ U).max(axis=(1, 2)) / np.abs(Mo).max(axis=(1, 2))
         assert res.max() < 1e-5, res.max()
-    xe = np.linalg.solve(M, b[ok][..., None])[..., 0]
-    xs = np.abs(xe).max(axis=1, keepdims=True)
-    assert np.max(np.abs(xc[ok] - xe) / xs) < 1e-3
+    xs = xs_all[ok]
+    # unrefined: the FP32 error; refined: converged (flag 0) to the FP64 solution
+    assert np.all(ir0[ok, 0] == 0) and np.all(ir0[ok, 1] == 0)
+    assert np.max(np.abs(x0[ok] - xe[ok]) / xs) < 1e-3
+    for x, ir in ((xc, irc), (xl, irl), (xh, irh)):
+        assert np.all(ir[ok, 1] == 0) and np.all(ir[ok, 0] >= 1) and np.all(ir[ok, 0] <= 3)
+        assert np.max(np.abs(x[ok] - xe[ok]) * ewt[ok]) < 1e-3
+    assert np.all(irc[idle] == -1.0)
 
 
 @pytest.mark.gpu
@@ -230,9 +247,9 @@ def test_cooperative_lu_matches_lane_lu(mech, wl):
         J[c, :, k] = 0.0
         J[c, k, k] = 2.0 ** 20
         pend[c] = 1
-    xc, sc, luc, pc = run_lu(G, mech, 1, J, gamma, pend, b)
-    xl, sl, lul, pl = run_lu(G, mech, 0, J, gamma, pend, b)
-    xh, sh, luh, ph = run_lu(Cl, mech, 0, J, gamma, pend, b)
+    xc, sc, luc, pc, _ = run_lu(G, mech, 1, J, gamma, pend, b)
+    xl, sl, lul, pl, _ = run_lu(G, mech, 0, J, gamma, pend, b)
+    xh, sh, luh, ph, _ = run_lu(Cl, mech, 0, J, gamma, pend, b)
     _dump(f"lu_{mech}", xc=xc, xl=xl, sc=sc, sl=sl, pc=pc, pl=pl)
     idle = pend == 0
     assert np.all(sc[idle] == -3.0) and np.all(luc[idle] == 7.0) and np.all(pc[idle] == -1)
