@@ -90,7 +90,24 @@ def kernel_model(st, nspec):
     }
 
 
-def kernel_roofline(st, nspec, traffic_bytes=None):
+def measured_traffic(kernel: str, form: str):
+    """HBM bytes per launch of `kernel` measured by the byte counters (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what it reads for a copy of known size), from the
+    committed summary for this workload; None when the summary has no such row."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_drm19_128_mix_{form}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == kernel:
+            try:
+                return float(cells[3])
+            except ValueError:
+                return None
+    return None
+
+
+def kernel_roofline(st, nspec, traffic_bytes=None, form="plain"):
     """Roofline of the kernel with the largest device time: algorithmic work per launch
     over its mean launch time (HIP events on the reactor's stream), against FP64 peak
     (the vector and matrix FP64 peaks of MI355X coincide) or HBM peak for k_consume."""
@@ -110,7 +127,7 @@ def kernel_roofline(st, nspec, traffic_bytes=None):
         achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
         line = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS}
-    line["traffic"] = traffic_bytes
+    line["traffic"] = traffic_bytes if traffic_bytes is not None else measured_traffic(name, form)
     line.update({
         "kernel": f"{name}<mech_drm19>: {KIND_DESC[name]}",
         "avg_launch_ms": avg_ms, "launches": launches,
@@ -221,7 +238,7 @@ def main(argv=None):
 
     roofline = None
     if not args.no_profile:
-        roofline = kernel_roofline(st, nspec, traffic_bytes=args.traffic_bytes)
+        roofline = kernel_roofline(st, nspec, traffic_bytes=args.traffic_bytes, form=best)
 
     cpu = None
     if not args.no_cpu:
