@@ -58,6 +58,8 @@ constexpr int HY2T_MAXIT = 50;
 // iterative refinement of FP32 factors: a sweep whose correction has WRMS norm (in
 // units of the error weights) at or below IR_TOL ends the refinement
 constexpr double IR_TOL = 1e-4;
+// Newton convergence threshold factor for attempts that start from a learned iterate
+constexpr double PRED_NLS_TIGHTEN = 0.01;
 constexpr double HY2T_RTOL = 1e-10;
 
 enum { FIRST_CALL, PREV_CONV_FAIL, PREV_ERR_FAIL };
@@ -72,7 +74,7 @@ struct Slots {
   static constexpr int ZN = 0;                       // LMAX * N1
   static constexpr int ACOR = ZN + LMAX * N1;
   static constexpr int EWT = ACOR + N1;
-  static constexpr int JS = EWT + N1;                 // saved Jacobian, row-major N1*N1
+  static constexpr int JS = EWT + N1;                 // Jacobian accumulation (copied to the matrix region)
   static constexpr int FY = JS + N1 * N1;             // species forcing (N1-1)
   static constexpr int TMP = FY + N1;                 // N1 scratch vector
   static constexpr int ATOL = TMP + N1;               // N1
@@ -87,13 +89,14 @@ struct Slots {
 };
 
 // Per-cell matrix region, cell-major (a cell's words are contiguous, so the 32 lanes
-// that factor or solve one cell read and write whole lines): the factors of
-// M = I - gamma J (row i holds row PERM[i] of M; FP64, or FP32 packed two per word),
-// the Newton right-hand side B and solution X, refinement information and PERM.
+// that factor or solve one cell read and write whole lines): the Jacobian, the factors
+// of M = I - gamma J (row i holds row PERM[i] of M; FP64, or FP32 packed), the Newton
+// right-hand side B and solution X, refinement information and PERM.
 template <int N1>
 struct MatSlots {
-  static constexpr int LU = 0;
-  static constexpr int B = N1 * N1;
+  static constexpr int JS = 0;                        // Jacobian copy, row-major (FP64)
+  static constexpr int LU = N1 * N1;
+  static constexpr int B = LU + N1 * N1;
   static constexpr int X = B + N1;
   static constexpr int IRS = X + N1;                  // refinement sweeps of the last solve
   static constexpr int IRF = IRS + 1;                 // 1: refinement did not converge
@@ -240,15 +243,14 @@ struct LUAcc<N1, float> {
 // one lane factors its cell's M = I - gamma J (partial pivoting, arithmetic in Real);
 // returns 0, or 1 + the first column without a nonzero pivot
 template <int N1, class Real>
-__device__ __noinline__ int lu_factor(const WS& ws, const CM& mat, const double gamma) {
-  using S = Slots<N1>;
+__device__ __noinline__ int lu_factor(const CM& mat, const double gamma) {
   using MS = MatSlots<N1>;
   using A = LUAcc<N1, Real>;
 #pragma unroll 1
   for (int i = 0; i < N1; ++i) {
     mat.perm(MS::PERM, i) = i;
 #pragma unroll
-    for (int j = 0; j < N1; ++j) A::at(mat, i * N1 + j) = (Real)((i == j ? 1.0 : 0.0) - gamma * ws[S::JS + i * N1 + j]);
+    for (int j = 0; j < N1; ++j) A::at(mat, i * N1 + j) = (Real)((i == j ? 1.0 : 0.0) - gamma * mat[MS::JS + i * N1 + j]);
   }
   int sing = 0;
 #pragma unroll 1
@@ -321,6 +323,7 @@ struct Lane {
   WS ws;
   IWS iws;
   CM mat;
+  const double* xs;   // this trip's Newton solution (matrix region, or its LDS copy)
   int qmax;
   double rtol;
   long mxstep;
@@ -788,7 +791,7 @@ struct Lane {
     ++nni;
     double x[N1];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) x[i] = mat[MS::X + i];
+    for (int i = 0; i < N1; ++i) x[i] = xs[i];
     if (lu_mode == LU_FP32_IR && !lu64) {
       nir += (long)mat[MS::IRS];
       if (mat[MS::IRF] != 0.0) {
@@ -814,10 +817,11 @@ struct Lane {
     for (int i = 0; i < N1; ++i) ws[S::ACOR + i] += x[i];
     const double del = wrms_reg(x);
     if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
-    const double dcon = del * fmin(1.0, crate) / tq[4];
-    // a learned iterate carries no convergence-rate history (the rate estimate belongs
-    // to the Nordsieck predictor), so its first correction is never taken as converged:
-    // convergence is judged from the second iteration on, with the rate it measures
+    // A learned iterate starts farther from the BDF solution than the Nordsieck
+    // prediction, so the iteration stops closer to the convergence threshold: its test
+    // is PRED_NLS_TIGHTEN times stricter, and its first correction (no convergence-rate
+    // history) is never taken as converged
+    const double dcon = del * fmin(1.0, crate) / (pred_state == 2 ? PRED_NLS_TIGHTEN * tq[4] : tq[4]);
     if (dcon <= 1.0 && !(m == 0 && pred_state == 2)) {
       // with a learned initial iterate acor is not the first correction alone
       acnrm = (m == 0 && pred_state != 2) ? del : wrms_ws(S::ACOR);
@@ -908,6 +912,7 @@ __device__ __forceinline__ void load_lane(Lane<M>& st, const ReactArgs& a, long 
   st.ws = w;
   st.iws = iw;
   st.mat = cell_mat<M>(a.mat, c);
+  st.xs = &st.mat[MatSlots<M::N1>::X];
   st.qmax = a.qmax;
   st.lu_mode = a.lu_mode;
   st.ir_sweeps = a.ir_sweeps;
@@ -938,6 +943,27 @@ __device__ __forceinline__ T lane_field(const WS& w, size_t off) {
   return v;
 }
 #define LANE_FIELD(M, w, f) lane_field<M, decltype(Lane<M>::f)>((w), offsetof(Lane<M>, f))
+
+// Copies between 64 lanes' rows in LDS (sm[l * LEN + i], lane l's cell cells[l], or -1)
+// and the cell-major region: consecutive lanes of a store or load handle consecutive
+// words of one cell's row, so each instruction touches a few whole lines instead of
+// one line per lane.  Called by all 64 threads of a one-wave block.
+template <int LEN>
+__device__ __forceinline__ void rows_to_cells(const double* sm, const long* cells, double* mat, long stride, int off) {
+  for (int e = threadIdx.x; e < 64 * LEN; e += 64) {
+    const int l = e / LEN, i = e - l * LEN;
+    const long c = cells[l];
+    if (c >= 0) mat[c * stride + off + i] = sm[l * LEN + i];
+  }
+}
+template <int LEN>
+__device__ __forceinline__ void cells_to_rows(double* sm, const long* cells, const double* mat, long stride, int off) {
+  for (int e = threadIdx.x; e < 64 * LEN; e += 64) {
+    const int l = e / LEN, i = e - l * LEN;
+    const long c = cells[l];
+    sm[l * LEN + i] = (c >= 0) ? mat[c * stride + off + i] : 0.0;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // the integration kernel
@@ -1039,6 +1065,8 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) k_eval(const ReactArg
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
   using MS = MatSlots<N1>;
+  __shared__ double sB[64 * N1];
+  __shared__ long scell[64];
   const long idx = (long)blockIdx.x * 64 + threadIdx.x;
   int iters = 0;
   bool newton = false;
@@ -1064,11 +1092,23 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) k_eval(const ReactArg
         // the Newton system's right-hand side gamma f - rl1 zn1 - acor
         newton = true;
         const double gamma = LANE_FIELD(M, w, gamma), rl1 = LANE_FIELD(M, w, rl1);
-        const CM mat = cell_mat<M>(a.mat, c);
 #pragma unroll
-        for (int i = 0; i < N1; ++i) mat[MS::B + i] = gamma * f[i] - rl1 * w[S::ZN + N1 + i] - w[S::ACOR + i];
+        for (int i = 0; i < N1; ++i) f[i] = gamma * f[i] - rl1 * w[S::ZN + N1 + i] - w[S::ACOR + i];
+        if (a.lu_lane) {
+          const CM mat = cell_mat<M>(a.mat, c);
+#pragma unroll
+          for (int i = 0; i < N1; ++i) mat[MS::B + i] = f[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < N1; ++i) sB[threadIdx.x * N1 + i] = f[i];
+        }
       }
     }
+  }
+  if (!a.lu_lane) {
+    scell[threadIdx.x] = newton ? c : -1;
+    __syncthreads();
+    rows_to_cells<N1>(sB, scell, a.mat, MS::STRIDE, MS::B);
   }
   wave_append(newton, (int)c, a.nlist, &a.counts[6]);
   if (a.counters) wave_add(&a.counters[CNT_HY2T], (unsigned long long)iters);
@@ -1080,21 +1120,52 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_SETUP_WAVES) k_setu
                                               const unsigned int* __restrict__ count) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
+  using MS = MatSlots<N1>;
+  __shared__ double sJ[64 * N1];
+  __shared__ long scell[64];
   const long idx = (long)blockIdx.x * 64 + threadIdx.x;
-  if (idx >= (long)*count) return;
-  const long c = list[idx];
-  const WS w = cell_ws<M>(a.ws, c);
-  Lane<M> st;
-  load_lane(st, a, c);
-  if (!st.needs_setup() || st.setup_pending) return;
-  double y[N1], f[N1];
+  long c = -1;
+  bool fresh = false;   // this lane evaluated a new Jacobian
+  if (idx < (long)*count) {
+    c = list[idx];
+    const WS w = cell_ws<M>(a.ws, c);
+    Lane<M> st;
+    load_lane(st, a, c);
+    if (st.needs_setup() && !st.setup_pending) {
+      double y[N1], f[N1];
 #pragma unroll
-  for (int i = 0; i < N1; ++i) {
-    y[i] = w[S::Y + i];
-    f[i] = w[S::F + i];
+      for (int i = 0; i < N1; ++i) {
+        y[i] = w[S::Y + i];
+        f[i] = w[S::F + i];
+      }
+      st.setup(y, f, w[S::TS]);
+      fresh = st.jcur;
+      store_lane(st, w);
+    }
   }
-  st.setup(y, f, w[S::TS]);
-  store_lane(st, w);
+  // the Jacobian (accumulated in the wave-tiled slots) goes to the matrix region, where
+  // the factorisation and refinement kernels read it row by row
+  if (a.lu_lane) {
+    if (fresh) {
+      const WS w = cell_ws<M>(a.ws, c);
+      const CM mat = cell_mat<M>(a.mat, c);
+#pragma unroll 1
+      for (int e = 0; e < N1 * N1; ++e) mat[MS::JS + e] = w[S::JS + e];
+    }
+  } else {
+    scell[threadIdx.x] = fresh ? c : -1;
+    const WS w = cell_ws<M>(a.ws, fresh ? c : 0);
+#pragma unroll 1
+    for (int r = 0; r < N1; ++r) {
+      __syncthreads();
+      if (fresh) {
+#pragma unroll
+        for (int j = 0; j < N1; ++j) sJ[threadIdx.x * N1 + j] = w[S::JS + r * N1 + j];
+      }
+      __syncthreads();
+      rows_to_cells<N1>(sJ, scell, a.mat, MS::STRIDE, MS::JS + r * N1);
+    }
+  }
 }
 
 // Learned initial iterate for the cells whose attempt requested one (pred_state 1):
@@ -1209,8 +1280,8 @@ __global__ void __launch_bounds__(64) k_lu_lane(const ReactArgs a, const int* __
   load_lane(st, a, c);
   if (!st.setup_pending) return;
   const bool fp32 = (st.lu_mode == LU_FP32_IR) && !st.lu64;
-  w[S::SING] = (double)(fp32 ? lu_factor<M::N1, float>(w, st.mat, st.gamma)
-                             : lu_factor<M::N1, double>(w, st.mat, st.gamma));
+  w[S::SING] = (double)(fp32 ? lu_factor<M::N1, float>(st.mat, st.gamma)
+                             : lu_factor<M::N1, double>(st.mat, st.gamma));
 }
 
 
@@ -1254,9 +1325,9 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
   const bool row = work && r < N1;
   Real m[N1];
   if (row) {
-    const WS w = cell_ws<M>(a.ws, c);
+    const CM mat = cell_mat<M>(a.mat, c);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) m[j] = (Real)((r == j ? 1.0 : 0.0) - gamma * w[S::JS + r * N1 + j]);
+    for (int j = 0; j < N1; ++j) m[j] = (Real)((r == j ? 1.0 : 0.0) - gamma * mat[MatSlots<N1>::JS + r * N1 + j]);
   } else {
 #pragma unroll
     for (int j = 0; j < N1; ++j) m[j] = (Real)0;
@@ -1350,7 +1421,7 @@ __global__ void __launch_bounds__(64) k_solve_lane(const ReactArgs a, const int*
       for (int i = 0; i < N1; ++i) {
         double v = b[i] - x[i];
 #pragma unroll
-        for (int j = 0; j < N1; ++j) v += gp * w[S::JS + i * N1 + j] * x[j];
+        for (int j = 0; j < N1; ++j) v += gp * mat[MS::JS + i * N1 + j] * x[j];
         r[i] = v;
       }
       lu_solve<N1, float>(mat, r, d);
@@ -1439,7 +1510,7 @@ __global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int*
 #pragma unroll
       for (int j = 0; j < N1; ++j) {
         const double xj = __shfl(x, j, 32);
-        if (row) res += gp * w[S::JS + r * N1 + j] * xj;
+        if (row) res += gp * mat[MS::JS + r * N1 + j] * xj;
       }
       const double d = subst(__shfl(res, pr, 32));
       x += d;
@@ -1466,6 +1537,8 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_co
   using S = Slots<N1>;
   const long idx = (long)blockIdx.x * 64 + threadIdx.x;
   const long n = (long)*count;
+  __shared__ double sX[64 * N1];
+  __shared__ long scell[64];
   bool keep = false, want_setup = false, want_pred = false;
   unsigned long long t_cells = 0, t_nst = 0, t_nfe = 0, t_nje = 0, t_nsetups = 0, t_nni = 0, t_ncfn = 0,
                      t_netf = 0, t_fail = 0, t_maxst = 0, t_ir = 0, t_lufb = 0, t_pred = 0, t_predfb = 0;
@@ -1474,10 +1547,19 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_co
     c = list[idx];
   }
   const WS w0 = cell_ws<M>(a.ws, c < 0 ? 0 : c);
-  if (c >= 0 && LANE_FIELD(M, w0, phase) != PH_NEED_CELL) {
+  const int phase0 = (c >= 0) ? LANE_FIELD(M, w0, phase) : (int)PH_NEED_CELL;
+  if (!a.lu_lane) {
+    // this trip's Newton solutions, read from the matrix region as whole lines
+    scell[threadIdx.x] = (phase0 == PH_NEWTON) ? c : -1;
+    __syncthreads();
+    cells_to_rows<N1>(sX, scell, a.mat, MatSlots<N1>::STRIDE, MatSlots<N1>::X);
+    __syncthreads();
+  }
+  if (phase0 != PH_NEED_CELL) {
     const WS w = w0;
     Lane<M> st;
     load_lane(st, a, c);
+    if (!a.lu_lane) st.xs = &sX[threadIdx.x * N1];
     {
       ++st.nfe;
       double f[N1];

@@ -74,7 +74,8 @@ __global__ void k_jac_lu(const double* y, const double* T, const double* fT, con
     double yy[N1];
     for (int i = 0; i < N1; ++i) yy[i] = y[c * N1 + i];
     jacobian<M>(yy, T[c], fT[c], w);
-    sg = lu_factor<N1, double>(w, mat, gamma[c]);
+    for (int e = 0; e < N1 * N1; ++e) mat[MatSlots<N1>::JS + e] = w[S::JS + e];
+    sg = lu_factor<N1, double>(mat, gamma[c]);
     double bb[N1], xx[N1];
     for (int i = 0; i < N1; ++i) bb[i] = b[c * N1 + i];
     lu_solve<N1>(mat, bb, xx);
@@ -300,7 +301,7 @@ __global__ void k_lu_prep(const double* J, const double* gamma, const int* pend,
   st.phase = PH_NEWTON;
   store_lane(st, w);
   for (int e = 0; e < N1 * N1; ++e) {
-    w[S::JS + e] = J[c * N1 * N1 + e];
+    mat[MS::JS + e] = J[c * N1 * N1 + e];
     mat[MS::LU + e] = 7.0;
   }
   for (int i = 0; i < N1; ++i) {
