@@ -1,13 +1,18 @@
 // Runs the reactor kernels' device code serially on the CPU (see hip/hip_runtime.h
 // in this directory) so tests can check the kernel logic against the oracle without a GPU.
 #define PELE_WAVES_PER_EU(n)
+#include <vector>
+
 #include "../../pelelm_amd/csrc/reactor_bdf.hip"
 
 thread_local Idx3 threadIdx, blockIdx, blockDim, gridDim;
 
 using namespace pelelm;
 
-static int g_lu_mode = LU_FP64, g_ir_sweeps = 3;
+static int g_lu_mode = LU_FP64, g_ir_sweeps = 3, g_pred = 0;
+// 1: attach a predictor whose network is all zeros (its iterate is the state at tn,
+// the "previous value" guess) -- the predictor's plumbing without a trained network
+extern "C" void emu_set_pred(int on) { g_pred = on; }
 // factor precision of the following emu_react calls (LU_FP64 = 0, LU_FP32_IR = 1)
 extern "C" void emu_set_lu(int mode, int ir_sweeps) {
   g_lu_mode = mode;
@@ -48,7 +53,23 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   a.lu_mode = g_lu_mode;
   a.ir_sweeps = g_ir_sweeps;
   a.counters = counters;
-  const hipError_t e = run_react(m, a, nullptr, 3 * 20000 + 100, trips, nullptr);
+  MlpDev md{};
+  std::vector<unsigned short> wz(2 * 64 * 128 * 8, 0);
+  std::vector<float> bz(4 * 128, 0.f), norm(256, 0.f);
+  for (int i = 0; i < 64; ++i) norm[64 + i] = norm[192 + i] = 1.f;
+  if (g_pred) {
+    const int N1 = N + 1;
+    md.n_in = N + 3;
+    md.n_out = N1;
+    md.kp[0] = 32; md.np[3] = 32;
+    for (int l = 0; l < 4; ++l) {
+      md.wt[l] = md.wk[l] = wz.data();
+      md.b[l] = bz.data();
+    }
+    md.norm = norm.data();
+    a.use_pred = 1;
+  }
+  const hipError_t e = run_react(m, a, nullptr, 3 * 20000 + 100, trips, nullptr, g_pred ? &md : nullptr);
   delete[] ws;
   delete[] iws;
   delete[] lists;

@@ -58,3 +58,32 @@ def test_emulated_kernels_match_oracle(emu, wl, mech, n, compact):
     assert np.max(np.abs(rY - ref["rY"]) / tol) < 1e-3
     assert np.max(np.abs(T - ref["T"])) < 1e-8
     assert trips.value >= int(ref["FC"].max())
+
+
+def test_predictor_plumbing_with_zero_network(emu):
+    """The learned-predictor path (iterate from the network, Nordsieck reference for the
+    error test, fallback on Newton failure) with a network of zeros, whose iterate is
+    the state at tn: every cell must finish, predictions and fallbacks are counted, and
+    the accuracy against the tight oracle stays at the plain form's level."""
+    d = cells.generate_workload("drm19_128_mix", ncell=256)
+    m = O.Mech("drm19")
+    ref = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    tight = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-12, atol=1e-20, max_steps=200000)
+    n = 256
+    rY, T, rE = d["rY"].copy(), d["T"].copy(), d["rE"].copy()
+    FC = np.zeros(n)
+    cnt = np.zeros(16, dtype=np.uint64)
+    trips = C.c_long(0)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    emu.emu_set_pred(1)
+    try:
+        rc = emu.emu_react(b"drm19", C.c_long(n), p(rY), p(T), p(rE), p(FC), C.c_double(d["dt"]),
+                           C.c_double(1e-6), C.c_double(1e-10), 4, 1, p(cnt), C.byref(trips))
+    finally:
+        emu.emu_set_pred(0)
+    assert rc == 0
+    assert cnt[8] == 0                      # no failed cell
+    assert cnt[13] > 0 and cnt[14] > 0      # predictions used, fallbacks taken
+    e_plain = np.abs(ref["T"] - tight["T"]).max()
+    assert np.abs(T - tight["T"]).max() <= 5.0 * e_plain + 0.05
+    assert np.all(FC > 0)
