@@ -405,40 +405,66 @@ struct Lane {
 #pragma unroll
         for (int i = 0; i < N1; ++i) zn(j - 1, i) -= zn(j, i);
   }
+  // The coefficient arrays l, tq and tau are only ever indexed with compile-time
+  // indices (loops fully unrolled over their extent and guarded by q; an element at
+  // index q is picked inside such a loop), so the lane state stays in registers: one
+  // run-time index would put the whole state object in scratch memory.
   __device__ void increase_bdf() {
-    for (int i = 0; i <= qmax; ++i) l[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < LMAX; ++i)
+      if (i <= qmax) l[i] = 0.0;
     l[2] = 1.0;
     double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = hscale;
     if (q > 1) {
-      for (int j = 1; j < q; ++j) {
-        hsum += tau[j + 1];
-        const double xi = hsum / hscale;
-        prod *= xi;
-        alpha0 -= 1.0 / (j + 1);
-        alpha1 += 1.0 / xi;
-        for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xiold + l[i - 1];
-        xiold = xi;
+#pragma unroll
+      for (int j = 1; j < LMAX - 1; ++j) {
+        if (j < q) {
+          hsum += tau[j + 1];
+          const double xi = hsum / hscale;
+          prod *= xi;
+          alpha0 -= 1.0 / (j + 1);
+          alpha1 += 1.0 / xi;
+#pragma unroll
+          for (int i = LMAX - 1; i >= 2; --i)
+            if (i <= j + 2) l[i] = l[i] * xiold + l[i - 1];
+          xiold = xi;
+        }
       }
     }
     const double A1 = (-alpha0 - alpha1) / prod;
 #pragma unroll
     for (int i = 0; i < N1; ++i) zn(L, i) = A1 * zn(qmax, i);
-    for (int j = 2; j <= q; ++j)
 #pragma unroll
-      for (int i = 0; i < N1; ++i) zn(j, i) += l[j] * zn(L, i);
+    for (int j = 2; j < LMAX; ++j)
+      if (j <= q) {
+        const double lj = l[j];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) zn(j, i) += lj * zn(L, i);
+      }
   }
   __device__ void decrease_bdf() {
-    for (int i = 0; i <= qmax; ++i) l[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < LMAX; ++i)
+      if (i <= qmax) l[i] = 0.0;
     l[2] = 1.0;
     double hsum = 0.0;
-    for (int j = 1; j <= q - 2; ++j) {
-      hsum += tau[j];
-      const double xi = hsum / hscale;
-      for (int i = j + 2; i >= 2; --i) l[i] = l[i] * xi + l[i - 1];
-    }
-    for (int j = 2; j < q; ++j)
 #pragma unroll
-      for (int i = 0; i < N1; ++i) zn(j, i) -= l[j] * zn(q, i);
+    for (int j = 1; j < LMAX - 2; ++j) {
+      if (j <= q - 2) {
+        hsum += tau[j];
+        const double xi = hsum / hscale;
+#pragma unroll
+        for (int i = LMAX - 1; i >= 2; --i)
+          if (i <= j + 2) l[i] = l[i] * xi + l[i - 1];
+      }
+    }
+#pragma unroll
+    for (int j = 2; j < LMAX; ++j)
+      if (j < q) {
+        const double lj = l[j];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) zn(j, i) -= lj * zn(q, i);
+      }
   }
   __device__ void adjust_order(int dq) {
     if (q == 2 && dq != 1) return;
@@ -447,29 +473,44 @@ struct Lane {
   __device__ void set_coeffs() {
     l[0] = l[1] = 1.0;
     double xi_inv = 1.0, xistar_inv = 1.0;
-    for (int i = 2; i <= q; ++i) l[i] = 0.0;
+#pragma unroll
+    for (int i = 2; i < LMAX; ++i)
+      if (i <= q) l[i] = 0.0;
     double alpha0 = -1.0, alpha0_hat = -1.0, hsum = h;
     if (q > 1) {
-      for (int j = 2; j < q; ++j) {
-        hsum += tau[j - 1];
-        xi_inv = h / hsum;
-        alpha0 -= 1.0 / j;
-        for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+#pragma unroll
+      for (int j = 2; j < LMAX; ++j) {
+        if (j < q) {
+          hsum += tau[j - 1];
+          xi_inv = h / hsum;
+          alpha0 -= 1.0 / j;
+#pragma unroll
+          for (int i = LMAX - 1; i >= 1; --i)
+            if (i <= j) l[i] += l[i - 1] * xi_inv;
+        }
       }
       alpha0 -= 1.0 / q;
       xistar_inv = -l[1] - alpha0;
-      hsum += tau[q - 1];
+#pragma unroll
+      for (int j = 2; j < LMAX; ++j)
+        if (j == q) hsum += tau[j - 1];
       xi_inv = h / hsum;
       alpha0_hat = -l[1] - xi_inv;
-      for (int i = q; i >= 1; --i) l[i] += l[i - 1] * xistar_inv;
+#pragma unroll
+      for (int i = LMAX - 1; i >= 1; --i)
+        if (i <= q) l[i] += l[i - 1] * xistar_inv;
     }
+    double lq = l[1];
+#pragma unroll
+    for (int i = 2; i < LMAX; ++i)
+      if (i == q) lq = l[i];
     const double A1 = 1.0 - alpha0_hat + alpha0;
     const double A2 = 1.0 + q * A1;
     tq[2] = fabs(A1 / (alpha0 * A2));
-    tq[5] = fabs(A2 * xistar_inv / (l[q] * xi_inv));
+    tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
     if (qwait == 1) {
       if (q > 1) {
-        const double C = xistar_inv / l[q];
+        const double C = xistar_inv / lq;
         const double A3 = alpha0 + 1.0 / q;
         const double A4 = alpha0_hat + xi_inv;
         const double Cpinv = (1.0 - A4 + A3) / A3;
@@ -477,7 +518,9 @@ struct Lane {
       } else {
         tq[1] = 1.0;
       }
-      hsum += tau[q];
+#pragma unroll
+      for (int j = 1; j <= LMAX; ++j)
+        if (j == q) hsum += tau[j];
       xi_inv = h / hsum;
       const double A5 = alpha0 - 1.0 / (q + 1);
       const double A6 = alpha0_hat - xi_inv;
@@ -727,14 +770,18 @@ struct Lane {
     }
     // complete the step
     ++nst;
-    for (int i = q; i >= 2; --i) tau[i] = tau[i - 1];
+#pragma unroll
+    for (int i = LMAX; i >= 2; --i)
+      if (i <= q) tau[i] = tau[i - 1];
     if (q == 1 && nst > 1) tau[2] = tau[1];
     tau[1] = h;
-    for (int j = 0; j <= q; ++j) {
-      const double lj = l[j];
 #pragma unroll
-      for (int i = 0; i < N1; ++i) zn(j, i) += lj * ws[S::ACOR + i];
-    }
+    for (int j = 0; j < LMAX; ++j)
+      if (j <= q) {
+        const double lj = l[j];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) zn(j, i) += lj * ws[S::ACOR + i];
+      }
     --qwait;
     if (qwait == 1 && q != qmax) {
 #pragma unroll
@@ -897,20 +944,47 @@ __device__ __forceinline__ CM cell_mat(double* mat, long c) {
   return CM{mat + c * (long)MatSlots<M::N1>::STRIDE};
 }
 
-// the lane state persists between the kernels of a trip as words of the workspace
+// The lane state persists between the kernels of a trip as words of the workspace, at
+// the byte offsets of the Lane<M> members.  It is loaded and stored member by member
+// (not as a block copy, which would pin the whole object in scratch memory), and the
+// per-call settings and pointers are set from the call instead of stored.
+#define PELE_LANE_PERSISTED(X)                                                                  \
+  X(tn) X(h) X(hprime) X(hscale) X(eta) X(etamax) X(gamma) X(gammap) X(gamrat) X(rl1) X(crate)  \
+  X(delp) X(acnrm) X(saved_tq5) X(q) X(qprime) X(L) X(qwait) X(nst) X(nfe) X(nje) X(nsetups)     \
+  X(nni) X(ncfn) X(netf) X(nstlp) X(nstlj) X(nstloc) X(jcur) X(saved_t) X(dsm) X(ncf) X(nef)     \
+  X(nflag) X(m) X(convfail) X(callSetup) X(hg) X(hub) X(hlb) X(hnew) X(count1) X(a1) X(a2)       \
+  X(teval) X(phase) X(status) X(rE0) X(fH) X(setup_pending) X(lu64) X(nir) X(nlufb) X(pred_state) \
+  X(npred) X(npredfb)
+#define PELE_LANE_ARRAYS(X) X(tq, 6) X(l, LMAX) X(tau, LMAX + 1)
+
+template <class T>
+__device__ __forceinline__ T word_get(const WS& w, int base, size_t off) {
+  const double word = w[base + (int)(off / 8)];
+  T v;
+  __builtin_memcpy(&v, reinterpret_cast<const char*>(&word) + off % 8, sizeof(T));
+  return v;
+}
+template <class T>
+__device__ __forceinline__ void word_put(const WS& w, int base, size_t off, const T& v) {
+  __builtin_memcpy(reinterpret_cast<char*>(&w[base + (int)(off / 8)]) + off % 8, &v, sizeof(T));
+}
+
 template <class M>
 __device__ __forceinline__ void load_lane(Lane<M>& st, const ReactArgs& a, long c) {
-  const WS w = cell_ws<M>(a.ws, c);
-  const IWS iw = cell_iws<M>(a.iws, c);
   using S = Slots<M::N1>;
-  constexpr int NW = (int)(sizeof(Lane<M>) / 8);
-  static_assert(sizeof(Lane<M>) % 8 == 0 && NW <= S::LANE_WORDS, "lane state does not fit its slots");
-  double tmp[NW];
-#pragma unroll
-  for (int i = 0; i < NW; ++i) tmp[i] = w[S::LANE + i];
-  __builtin_memcpy(&st, tmp, sizeof(Lane<M>));
+  static_assert(sizeof(Lane<M>) % 8 == 0 && (int)(sizeof(Lane<M>) / 8) <= S::LANE_WORDS,
+                "lane state does not fit its slots");
+  const WS w = cell_ws<M>(a.ws, c);
+#define PELE_GET(f) st.f = word_get<decltype(st.f)>(w, S::LANE, offsetof(Lane<M>, f));
+#define PELE_GETA(f, n)                                                                        \
+  _Pragma("unroll") for (int i_ = 0; i_ < (n); ++i_) st.f[i_] =                                \
+      word_get<double>(w, S::LANE, offsetof(Lane<M>, f) + i_ * sizeof(double));
+  PELE_LANE_PERSISTED(PELE_GET)
+  PELE_LANE_ARRAYS(PELE_GETA)
+#undef PELE_GET
+#undef PELE_GETA
   st.ws = w;
-  st.iws = iw;
+  st.iws = cell_iws<M>(a.iws, c);
   st.mat = cell_mat<M>(a.mat, c);
   st.xs = &st.mat[MatSlots<M::N1>::X];
   st.qmax = a.qmax;
@@ -924,11 +998,14 @@ __device__ __forceinline__ void load_lane(Lane<M>& st, const ReactArgs& a, long 
 template <class M>
 __device__ __forceinline__ void store_lane(const Lane<M>& st, const WS& w) {
   using S = Slots<M::N1>;
-  constexpr int NW = (int)(sizeof(Lane<M>) / 8);
-  double tmp[NW];
-  __builtin_memcpy(tmp, &st, sizeof(Lane<M>));
-#pragma unroll
-  for (int i = 0; i < NW; ++i) w[S::LANE + i] = tmp[i];
+#define PELE_PUT(f) word_put(w, S::LANE, offsetof(Lane<M>, f), st.f);
+#define PELE_PUTA(f, n)                                                                        \
+  _Pragma("unroll") for (int i_ = 0; i_ < (n); ++i_)                                           \
+      word_put(w, S::LANE, offsetof(Lane<M>, f) + i_ * sizeof(double), st.f[i_]);
+  PELE_LANE_PERSISTED(PELE_PUT)
+  PELE_LANE_ARRAYS(PELE_PUTA)
+#undef PELE_PUT
+#undef PELE_PUTA
 }
 
 // a field of the lane state (at byte offset `off` of Lane<M>), read straight from its
