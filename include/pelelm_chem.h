@@ -69,6 +69,12 @@ typedef struct {
   double kernel_ms[MI_NKERNEL_KINDS];
   long kernel_launches[MI_NKERNEL_KINDS];
   long kernel_cells[MI_NKERNEL_KINDS];
+  /* how the trips ran: schedule 0 plain / 1 queue, graph_trips 0 host-driven or K trips
+     per captured graph; with mi.schedule = auto, the trial times (ms) of plain, queue,
+     plain + graphs and queue + graphs for this box size (0 when not tuned) */
+  long schedule_used;
+  long graph_trips_used;
+  double tune_ms[4];
 } mi_chem_stats;
 
 /* ReactorBase::create(name) -- Source/PeleLM.cpp:706.
@@ -86,9 +92,13 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
      ode.rtol, ode.atol, cvode.max_order, cvode.max_num_steps (default 20000),
      cvode.solve_type (denseAJ_direct), peleLM.use_typ_vals_chem (0/1),
      mi.mechanism (drm19 | LiDryer), mi.schedule (queue: finished cells leave the work
-     lists, default | plain: fixed cell-to-lane mapping), mi.lu (fp64: 32 lanes
-     factorise one cell's matrix, default | fp64_lane: one lane per cell), mi.ir_sweeps,
-     mi.on_fail (continue | abort), mi.profile (0 | 1: per-kernel event timing). */
+     lists, default | plain: fixed cell-to-lane mapping for the evaluation and step
+     kernels | auto: the first call of each box size times both, with and without graph
+     capture, on a copy of its cells and keeps the fastest), mi.graph_trips (0: the host
+     issues every trip, default | K even: K trips per captured HIP graph), mi.lu (fp64 |
+     fp32_ir: FP32 factors with FP64-residual refinement), mi.ir_sweeps (3), mi.lu_form
+     (coop: 32 lanes per cell, default | lane: one lane per cell), mi.on_fail
+     (continue | abort), mi.profile (0 | 1: per-kernel event timing). */
 int mi_chem_set(void* reactor, const char* key, const char* value);
 
 /* ReactorBase::set_typ_vals_ode -- Source/PeleLM.cpp:1497-1516: n = nspec + 1 typical

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -39,6 +40,15 @@ struct Reactor {
   int ir_sweeps = 3;
   int lu_lane = 0;
   void* predictor = nullptr;    // attached network (mi_chem_attach_predictor), not owned
+  int graph_trips = 0;          // 0: host-driven trips; K (even): graphs of K trips
+  hipStream_t capture_stream = nullptr;
+  // mi.schedule = auto: the configuration chosen per box size, and the trial timings
+  struct Tuned {
+    int schedule, graph_trips;
+    double ms[4];
+  };
+  std::map<long, Tuned> tuned;
+  int last_schedule = SCHED_COMPACT, last_graph = 0;
   bool on_fail_continue = true;
   bool profile = false;
   KernelProf prof{};
@@ -124,6 +134,72 @@ bool parse_long(const char* s, long* out) {
   return true;
 }
 
+// mi.schedule = auto: time the plain and compacted schedules, each with host-driven and
+// graph-captured trips, on a flat copy of this call's cells (the call's own arrays are
+// untouched) and remember the fastest for this box size.  One-time cost per box size:
+// four integrations of the box.
+int autotune(Reactor* r, const ReactArgs& a0, hipStream_t stream, long max_trips) {
+  const int N = mech_nspec(r->mech);
+  const long n = a0.ncell;
+  double *rY = nullptr, *T = nullptr, *rE = nullptr, *FC = nullptr, *cY = nullptr, *cT = nullptr, *cE = nullptr;
+  struct Free {
+    double** p[7];
+    ~Free() {
+      for (auto q : p)
+        if (*q) (void)hipFree(*q);
+    }
+  } fr{{&rY, &T, &rE, &FC, &cY, &cT, &cE}};
+  HIPCHK(r, hipMalloc(&rY, sizeof(double) * N * n));
+  HIPCHK(r, hipMalloc(&cY, sizeof(double) * N * n));
+  for (double** p : {&T, &rE, &FC, &cT, &cE}) HIPCHK(r, hipMalloc(p, sizeof(double) * n));
+  HIPCHK(r, launch_gather_state(N, a0, cY, cT, cE, stream));
+  ReactArgs a = a0;
+  auto flat = [n](double* p) {
+    ArrView v{};
+    v.p = p;
+    v.js = n;
+    v.ks = n;
+    v.ns = n;
+    return v;
+  };
+  a.rY = flat(rY);
+  a.T = flat(T);
+  a.rE = flat(rE);
+  a.FC = flat(FC);
+  a.rYsrc = ArrView{};
+  a.rEsrc = ArrView{};
+  a.mask = IntView{};
+  a.nx = (int)n;
+  a.ny = 1;
+  a.use_pred = 0;
+  const int sched[4] = {SCHED_PLAIN, SCHED_COMPACT, SCHED_PLAIN, SCHED_COMPACT};
+  const int gk[4] = {0, 0, 8, 8};
+  Reactor::Tuned t{};
+  int best = 0;
+  for (int v = 0; v < 4; ++v) {
+    HIPCHK(r, hipMemcpyAsync(rY, cY, sizeof(double) * N * n, hipMemcpyDeviceToDevice, stream));
+    HIPCHK(r, hipMemcpyAsync(T, cT, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    HIPCHK(r, hipMemcpyAsync(rE, cE, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    a.compact = (sched[v] == SCHED_COMPACT) ? 1 : 0;
+    TripPlan plan;
+    plan.graph_trips = gk[v];
+    plan.capture_stream = r->capture_stream;
+    long trips = 0;
+    HIPCHK(r, hipEventRecord(r->ev0, stream));
+    HIPCHK(r, run_react(r->mech, a, stream, max_trips, &trips, nullptr, nullptr, plan));
+    HIPCHK(r, hipEventRecord(r->ev1, stream));
+    HIPCHK(r, hipEventSynchronize(r->ev1));
+    float ms = 0.f;
+    HIPCHK(r, hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    t.ms[v] = ms;
+    if (ms < t.ms[best]) best = v;
+  }
+  t.schedule = sched[best];
+  t.graph_trips = gk[best];
+  r->tuned[n] = t;
+  return MI_OK;
+}
+
 int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream) {
   if (!r->inited) return fail(r, MI_ESTATE, "react called before init");
   if (!dt || !(*dt > 0.0)) return fail(r, MI_EINVAL, "dt must be positive");
@@ -140,6 +216,7 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   a.ir_sweeps = r->ir_sweeps;
   a.lu_lane = r->lu_lane;
   a.compact = (r->schedule == SCHED_COMPACT) ? 1 : 0;
+  int graph_trips = r->graph_trips;
   a.cell_status = nullptr;
   r->last_ncell = a.ncell;
   if (a.ncell == 0) {
@@ -158,19 +235,37 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   a.nlist = r->lists + (size_t)5 * r->ws_cells;
   a.mat = r->mat;
   a.use_pred = (r->predictor && !r->null_reactor) ? 1 : 0;
-  if (a.use_pred && !a.compact) return fail(r, MI_EINVAL, "the learned predictor needs mi.schedule = queue");
   a.counts = r->counts;
   a.counters = r->counters;
+  // every step takes at least one trip; 3 per step bounds retries generously
+  const long max_trips = 3 * r->max_steps + 100;
+  if (graph_trips > 0 || r->schedule == SCHED_AUTO) {
+    if (!r->capture_stream) HIPCHK(r, hipStreamCreateWithFlags(&r->capture_stream, hipStreamNonBlocking));
+  }
+  if (r->schedule == SCHED_AUTO && !r->null_reactor) {
+    auto it = r->tuned.find(a.ncell);
+    if (it == r->tuned.end()) {
+      rc = autotune(r, a, stream, max_trips);
+      if (rc) return rc;
+      it = r->tuned.find(a.ncell);
+    }
+    a.compact = (it->second.schedule == SCHED_COMPACT) ? 1 : 0;
+    graph_trips = it->second.graph_trips;
+  }
+  if (a.use_pred && !a.compact) return fail(r, MI_EINVAL, "the learned predictor needs mi.schedule = queue");
+  r->last_schedule = a.compact ? SCHED_COMPACT : SCHED_PLAIN;
+  r->last_graph = (r->profile || a.use_pred) ? 0 : graph_trips;
   HIPCHK(r, launch_counters_reset(r->counters, stream));
   HIPCHK(r, hipEventRecord(r->ev0, stream));
   if (r->null_reactor) {
     HIPCHK(r, launch_null(N, a, stream));
   } else {
-    // every step takes at least one trip; 3 per step bounds retries generously
-    const long max_trips = 3 * r->max_steps + 100;
     r->prof = KernelProf{};
+    TripPlan plan;
+    plan.graph_trips = graph_trips;
+    plan.capture_stream = r->capture_stream;
     HIPCHK(r, run_react(r->mech, a, stream, max_trips, &r->last_trips, r->profile ? &r->prof : nullptr,
-                        mlp_device(r->predictor)));
+                        mlp_device(r->predictor), plan));
   }
   HIPCHK(r, hipEventRecord(r->ev1, stream));
   r->last_stream = stream;
@@ -275,7 +370,11 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
   } else if (k == "mi.schedule") {
     if (v == "plain") r->schedule = SCHED_PLAIN;
     else if (v == "queue") r->schedule = SCHED_COMPACT;
-    else return fail(r, MI_EINVAL, "mi.schedule must be plain or queue");
+    else if (v == "auto") r->schedule = SCHED_AUTO;
+    else return fail(r, MI_EINVAL, "mi.schedule must be plain, queue or auto");
+  } else if (k == "mi.graph_trips") {
+    if (!parse_long(value, &l) || l < 0 || l > 64 || (l & 1)) return fail(r, MI_EINVAL, "mi.graph_trips must be 0 or even up to 64");
+    r->graph_trips = (int)l;
   } else if (k == "mi.lu") {
     if (v == "fp64") r->lu_mode = LU_FP64;
     else if (v == "fp32_ir") r->lu_mode = LU_FP32_IR;
@@ -403,6 +502,11 @@ int mi_chem_get_stats(void* reactor, mi_chem_stats* out) {
     out->kernel_launches[kk] = r->prof.launches[kk];
     out->kernel_cells[kk] = r->prof.cells[kk];
   }
+  out->schedule_used = r->last_schedule;
+  out->graph_trips_used = r->last_graph;
+  auto it = r->tuned.find(r->last_ncell);
+  if (it != r->tuned.end())
+    for (int v = 0; v < 4; ++v) out->tune_ms[v] = it->second.ms[v];
   if (out->nfail > 0 && !r->on_fail_continue) return fail(r, MI_ECELLFAIL, "cells failed to integrate");
   return MI_OK;
 }
@@ -443,6 +547,7 @@ int mi_chem_close(void* reactor) {
   if (r->counters) (void)hipFree(r->counters);
   if (r->ev0) (void)hipEventDestroy(r->ev0);
   if (r->ev1) (void)hipEventDestroy(r->ev1);
+  if (r->capture_stream) (void)hipStreamDestroy(r->capture_stream);
   delete r;
   return MI_OK;
 }

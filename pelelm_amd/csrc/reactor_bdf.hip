@@ -1296,8 +1296,8 @@ __global__ void __launch_bounds__(64) k_lu_lane(const ReactArgs a, const int* __
 // the pending cells whose factor precision is Real (FP32 under LU_FP32_IR unless the
 // cell has fallen back to FP64).
 template <class M, class Real>
-__global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __restrict__ list,
-                                                const unsigned int* __restrict__ count) {
+__device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __restrict__ list, const long n,
+                                             const long blk) {
   constexpr int N1 = M::N1;
   static_assert(N1 <= 32, "the cooperative LU holds one row per lane of a 32-lane half-wave");
   using S = Slots<N1>;
@@ -1306,13 +1306,7 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
   __shared__ Real prow[2][N1];
   const int lane = threadIdx.x;
   const int half = lane >> 5, r = lane & 31;
-  // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous range of
-  // list entries so neighbouring cells (which share workspace lines) meet in one L2.
-  // The grid is a multiple of 8, so this is a permutation of the blocks.
-  const unsigned per_xcd = gridDim.x / 8;
-  const long blk = (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   const long idx = blk * 2 + half;
-  const long n = (long)*count;
   long c = -1;
   bool work = false;
   double gamma = 0.0;
@@ -1379,6 +1373,23 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
     mat.perm(MatSlots<N1>::PERM, my_step) = r;
     if (r == 0) w[S::SING] = (double)sing;
   }
+  __syncthreads();   // prow is reused by the next pair of this block
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin: each XCD gets contiguous ranges of
+// list entries, so neighbouring cells (whose tiles share lines) meet in one L2.  The
+// grid is a multiple of 8 (so the mapping is a permutation) and strides over the list,
+// so a fixed grid serves any count (as a captured graph needs).
+__device__ __forceinline__ long xcd_block() {
+  const unsigned per_xcd = gridDim.x / 8;
+  return (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+}
+
+template <class M, class Real>
+__global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __restrict__ list,
+                                                const unsigned int* __restrict__ count) {
+  const long n = (long)*count;
+  for (long blk = xcd_block(); blk * 2 < n; blk += gridDim.x) lu_coop_pair<M, Real>(a, list, n, blk);
 }
 
 // true when the cell's Newton iteration proceeds this trip (not a singular set-up)
@@ -1449,8 +1460,8 @@ __global__ void __launch_bounds__(64) k_solve_lane(const ReactArgs a, const int*
 // cells whose factor precision is Real.  No barriers: the two halves of a wave may
 // take different numbers of refinement sweeps.
 template <class M, class Real>
-__global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int* __restrict__ list,
-                                                   const unsigned int* __restrict__ count) {
+__device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* __restrict__ list, const long n,
+                                                const long blk) {
   constexpr int N1 = M::N1;
   static_assert(N1 <= 32, "one row per lane of a 32-lane half-wave");
   using S = Slots<N1>;
@@ -1458,12 +1469,10 @@ __global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int*
   using A = LUAcc<N1, Real>;
   constexpr bool IS32 = sizeof(Real) == 4;
   const int half = threadIdx.x >> 5, r = threadIdx.x & 31;
-  const unsigned per_xcd = gridDim.x / 8;
-  const long blk = (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   const long idx = blk * 2 + half;
   bool work = false;
   long c = 0;
-  if (idx < (long)*count) {
+  if (idx < n) {
     c = list[idx];
     const WS w = cell_ws<M>(a.ws, c);
     const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
@@ -1528,6 +1537,13 @@ __global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int*
     }
   }
   if (row) mat[MS::X + r] = x;
+}
+
+template <class M, class Real>
+__global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int* __restrict__ list,
+                                                   const unsigned int* __restrict__ count) {
+  const long n = (long)*count;
+  for (long blk = xcd_block(); blk * 2 < n; blk += gridDim.x) solve_coop_pair<M, Real>(a, list, n, blk);
 }
 
 // advance every active cell's state machine by this trip's evaluation
@@ -1762,13 +1778,20 @@ __global__ void null_kernel(const ReactArgs a, int nspec) {
     if (e_ != hipSuccess) return e_;    \
   } while (0)
 
+// blocks for the cooperative kernels (two cells per block) handling up to n cells: a
+// multiple of 8, at most 32 resident blocks per CU of the 256
+inline unsigned coop_grid(long n) {
+  const long want = ((n + 1) / 2 + 7) / 8 * 8;
+  return (unsigned)(want < 8192 ? (want > 0 ? want : 8) : 8192);
+}
+
 template <class M>
 hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* count, long n, hipStream_t stream) {
   if (a.lu_lane) {
     hipLaunchKernelGGL(k_lu_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, list, count);
     return hipGetLastError();
   }
-  const dim3 grid((unsigned)(((n + 1) / 2 + 7) / 8 * 8));   // two cells per block, multiple of 8 blocks
+  const dim3 grid(coop_grid(n));
   if (a.lu_mode == LU_FP32_IR) {
     hipLaunchKernelGGL((k_lu_coop<M, float>), grid, dim3(64), 0, stream, a, list, count);
     const hipError_t e = hipGetLastError();
@@ -1786,7 +1809,7 @@ hipError_t launch_solve(const ReactArgs& a, long n, hipStream_t stream) {
                        &a.counts[6]);
     return hipGetLastError();
   }
-  const dim3 grid((unsigned)(((n + 1) / 2 + 7) / 8 * 8));
+  const dim3 grid(coop_grid(n));
   if (a.lu_mode == LU_FP32_IR) {
     hipLaunchKernelGGL((k_solve_coop<M, float>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
     const hipError_t e = hipGetLastError();
@@ -1850,11 +1873,85 @@ hipError_t launch_predict(const ReactArgs& a, const MlpDev& w, unsigned np, hipS
   return hipGetLastError();
 }
 
+// One trip with grids sized for every cell (kernels skip beyond their list counts), as
+// captured into a graph: evaluation, set-up, factorisation, solves, step logic.
+template <class M>
+hipError_t enqueue_fixed_trip(const ReactArgs& a, int cur, hipStream_t s) {
+  const long n = a.ncell;
+  const unsigned nb = (unsigned)((n + 63) / 64);
+  const int nxt = 1 - cur;
+  if (a.compact)
+    hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, s, a.counts, nxt, 2 + nxt);
+  else
+    hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, s, a.counts, 5, 2 + nxt);
+  PELE_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_zero_two, dim3(1), dim3(64), 0, s, a.counts, 6, 6);
+  PELE_TRY(hipGetLastError());
+  const int* act = a.compact ? a.list[cur] : a.list[1];
+  const unsigned* act_n = a.compact ? &a.counts[cur] : &a.counts[4];
+  hipLaunchKernelGGL(k_eval<M>, dim3(nb), dim3(64), 0, s, a, act, act_n);
+  PELE_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_setup<M>, dim3(nb), dim3(64), 0, s, a, a.slist[cur], &a.counts[2 + cur]);
+  PELE_TRY(hipGetLastError());
+  PELE_TRY(launch_lu<M>(a, a.slist[cur], &a.counts[2 + cur], n, s));
+  PELE_TRY(launch_solve<M>(a, n, s));
+  if (a.compact)
+    hipLaunchKernelGGL(k_consume<M>, dim3(nb), dim3(64), 0, s, a, act, act_n, a.list[nxt], &a.counts[nxt],
+                       a.slist[nxt], &a.counts[2 + nxt]);
+  else
+    hipLaunchKernelGGL(k_consume<M>, dim3(nb), dim3(64), 0, s, a, act, act_n, a.list[0], &a.counts[5],
+                       a.slist[nxt], &a.counts[2 + nxt]);
+  return hipGetLastError();
+}
+
+// trips issued as a captured graph of plan.graph_trips trips, relaunched until done
+template <class M>
+hipError_t run_trips_graph(const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out,
+                           const TripPlan& plan) {
+  const int K = plan.graph_trips;
+  if (K <= 0 || (K & 1) || !plan.capture_stream) return hipErrorInvalidValue;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  PELE_TRY(hipStreamBeginCapture(plan.capture_stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (int t = 0; t < K && e == hipSuccess; ++t) e = enqueue_fixed_trip<M>(a, t & 1, plan.capture_stream);
+  const hipError_t ec = hipStreamEndCapture(plan.capture_stream, &g);
+  if (e != hipSuccess || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return e != hipSuccess ? e : ec;
+  }
+  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  long trip = 0;
+  unsigned int h_counts[8];
+  while (e == hipSuccess && trip < max_trips) {
+    e = hipMemcpyAsync(h_counts, a.counts, sizeof(h_counts), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) break;
+    if ((a.compact ? h_counts[0] : h_counts[5]) == 0) break;
+    e = hipGraphLaunch(ge, stream);
+    trip += K;
+  }
+  if (ge) (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  PELE_TRY(e);
+  if (trip >= max_trips) {
+    const unsigned nb = (unsigned)((a.ncell + 63) / 64);
+    if (a.compact)
+      hipLaunchKernelGGL(k_abandon<M>, dim3(nb), dim3(64), 0, stream, a, a.list[0], &a.counts[0]);
+    else
+      hipLaunchKernelGGL(k_abandon<M>, dim3(nb), dim3(64), 0, stream, a, a.list[1], &a.counts[4]);
+    PELE_TRY(hipGetLastError());
+  }
+  if (trips_out) *trips_out = trip;
+  return hipSuccess;
+}
+
 template <class M>
 hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* trips_out, KernelProf* prof,
-                       const MlpDev* pred) {
+                       const MlpDev* pred, const TripPlan& plan) {
   const long n = a.ncell;
   if (a.use_pred && (!pred || !a.compact || !a.plist)) return hipErrorInvalidValue;
+  const bool graph = plan.graph_trips > 0 && !prof && !a.use_pred;
   Timer* tmp = new Timer();
   tmp->prof = prof;
   tmp->stream = stream;
@@ -1874,6 +1971,7 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
     hipLaunchKernelGGL(k_plain_counts, dim3(1), dim3(64), 0, stream, a.counts, (unsigned int)n);
     PELE_TRY(hipGetLastError());
   }
+  if (graph) return run_trips_graph<M>(a, stream, max_trips, trips_out, plan);
   int cur = 0;
   long trip = 0;
   unsigned int h_counts[8];
@@ -2023,10 +2121,10 @@ long mat_stride(int mech) {
 }
 
 hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out,
-                     KernelProf* prof, const MlpDev* pred) {
+                     KernelProf* prof, const MlpDev* pred, TripPlan plan) {
   switch (mech) {
-    case MECH_LIDRYER: return run_react_t<mech_LiDryer>(a, stream, max_trips, trips_out, prof, pred);
-    case MECH_DRM19: return run_react_t<mech_drm19>(a, stream, max_trips, trips_out, prof, pred);
+    case MECH_LIDRYER: return run_react_t<mech_LiDryer>(a, stream, max_trips, trips_out, prof, pred, plan);
+    case MECH_DRM19: return run_react_t<mech_drm19>(a, stream, max_trips, trips_out, prof, pred, plan);
     default: return hipErrorInvalidValue;
   }
 }
@@ -2042,6 +2140,24 @@ hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T,
 
 hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t stream) {
   hipLaunchKernelGGL(counters_reset_kernel, dim3(1), dim3(64), 0, stream, counters);
+  return hipGetLastError();
+}
+
+// flat structure-of-arrays copy of a call's rY, T and rE (cell fastest), for trial runs
+__global__ void gather_state_kernel(const ReactArgs a, int nspec, double* __restrict__ rY, double* __restrict__ T,
+                                    double* __restrict__ rE) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncell) return;
+  int i, j, k;
+  cell_ijk(a, c, i, j, k);
+  for (int s = 0; s < nspec; ++s) rY[s * a.ncell + c] = a.rY.p[cell_off(a.rY, i, j, k) + s * a.rY.ns];
+  T[c] = a.T.p[cell_off(a.T, i, j, k)];
+  rE[c] = a.rE.p[cell_off(a.rE, i, j, k)];
+}
+
+hipError_t launch_gather_state(int nspec, const ReactArgs& a, double* rY, double* T, double* rE, hipStream_t stream) {
+  hipLaunchKernelGGL(gather_state_kernel, dim3((unsigned)((a.ncell + 255) / 256)), dim3(256), 0, stream, a, nspec,
+                     rY, T, rE);
   return hipGetLastError();
 }
 

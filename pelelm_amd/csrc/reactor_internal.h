@@ -23,7 +23,9 @@ struct IntView {
   int ox, oy, oz;
 };
 
-enum Schedule : int { SCHED_PLAIN = 0, SCHED_COMPACT = 1 };
+// SCHED_AUTO: the first call of each box size times the plain and compacted schedules,
+// each host-driven and graph-captured, on a copy of its cells and keeps the fastest
+enum Schedule : int { SCHED_PLAIN = 0, SCHED_COMPACT = 1, SCHED_AUTO = 2 };
 // LU_FP64: FP64 factors of I - gamma J; LU_FP32_IR: FP32 factors with FP64-residual
 // iterative refinement of every solve (a cell whose refinement fails falls back to FP64)
 enum LuMode : int { LU_FP64 = 0, LU_FP32_IR = 1 };
@@ -91,12 +93,24 @@ struct KernelProf {
   long cells[NKK];
 };
 struct MlpDev;
+// How the trips are issued: graph_trips = 0 -> the host launches every kernel of every
+// trip and reads the list counts after each; graph_trips = K (even) -> K trips are
+// captured once per call into a HIP graph (on capture_stream, which must not be the
+// null stream) and the graph is launched on `stream` until no cell is active, reading
+// the counts once per K trips.  Per-kernel timing and the learned predictor need the
+// host-driven form.
+struct TripPlan {
+  int graph_trips = 0;
+  hipStream_t capture_stream = nullptr;
+};
 // pred: the learned predictor's device view, required when a.use_pred = 1
 hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out,
-                     KernelProf* prof, const MlpDev* pred = nullptr);
+                     KernelProf* prof, const MlpDev* pred = nullptr, TripPlan plan = TripPlan{});
 hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t stream);
 hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T, const int* mask, double* rYdot,
                         long ncell, hipStream_t stream);
 hipError_t launch_null(int nspec, const ReactArgs& a, hipStream_t stream);
+// copy a call's rY [nspec][ncell], T and rE [ncell] into flat device arrays
+hipError_t launch_gather_state(int nspec, const ReactArgs& a, double* rY, double* T, double* rE, hipStream_t stream);
 
 }  // namespace pelelm

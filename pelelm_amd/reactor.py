@@ -40,7 +40,8 @@ class MiStats(C.Structure):
                 ("nlu_fallback", C.c_long), ("npred_used", C.c_long), ("npred_fallback", C.c_long),
                 ("ms_kernel", C.c_double), ("ntrips", C.c_long), ("nhy2t_iters", C.c_long),
                 ("kernel_ms", C.c_double * 6),
-                ("kernel_launches", C.c_long * 6), ("kernel_cells", C.c_long * 6)]
+                ("kernel_launches", C.c_long * 6), ("kernel_cells", C.c_long * 6),
+                ("schedule_used", C.c_long), ("graph_trips_used", C.c_long), ("tune_ms", C.c_double * 4)]
 
 
 # Every exported entry point of include/pelelm_chem.h.
@@ -181,7 +182,7 @@ class ReactorBase:
         st = MiStats()
         rc = lib().mi_chem_get_stats(self._h, C.byref(st))
         out = {f: getattr(st, f) for f, _ in MiStats._fields_}
-        for f in ("kernel_ms", "kernel_launches", "kernel_cells"):
+        for f in ("kernel_ms", "kernel_launches", "kernel_cells", "tune_ms"):
             out[f] = list(out[f])
         out["rc"] = rc
         return out

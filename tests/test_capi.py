@@ -56,7 +56,9 @@ def test_init_rejects_e_reactor():
 
 @pytest.mark.parametrize("key,value", [("ode.rtol", "-1"), ("ode.rtol", "abc"), ("cvode.max_order", "7"),
                                        ("cvode.solve_type", "GMRES"), ("mi.mechanism", "gri30"),
-                                       ("mi.schedule", "fifo"), ("no.such.key", "1")])
+                                       ("mi.schedule", "fifo"), ("mi.graph_trips", "3"), ("mi.graph_trips", "-2"),
+                                       ("mi.graph_trips", "66"), ("mi.lu", "fp16"), ("mi.lu_form", "warp"),
+                                       ("no.such.key", "1")])
 def test_set_rejects_bad_keys(key, value):
     r = reactor.ReactorBase.create("ReactorCvode")
     with pytest.raises(reactor.ReactorError):
@@ -68,7 +70,9 @@ def test_set_accepts_reference_deck_keys():
     r = reactor.ReactorBase.create("ReactorCvode")
     for k, v in [("ode.rtol", "1.0e-6"), ("ode.atol", "1.0e-5"), ("cvode.solve_type", "denseAJ_direct"),
                  ("cvode.max_order", "4"), ("peleLM.use_typ_vals_chem", "1"), ("mi.mechanism", "LiDryer"),
-                 ("mi.schedule", "plain"), ("mi.schedule", "queue")]:
+                 ("mi.schedule", "plain"), ("mi.schedule", "queue"), ("mi.schedule", "auto"),
+                 ("mi.graph_trips", "8"), ("mi.graph_trips", "0"), ("mi.lu", "fp32_ir"), ("mi.lu", "fp64"),
+                 ("mi.lu_form", "lane"), ("mi.lu_form", "coop"), ("mi.ir_sweeps", "2")]:
         r.set(k, v)
     assert r.nspec == 9
     r.set_typ_vals_ode([1e-3] * 9 + [1500.0])

@@ -189,3 +189,49 @@ def test_react_box_on_array4_views_with_ghost_cells():
     tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
     assert (np.abs(got[:N] - ref["rY"]) / tol).max() < 10.0
     assert np.abs(got[N + 1] - ref["T"]).max() < 1e-2
+
+
+@pytest.mark.parametrize("schedule", ["plain", "queue"])
+def test_graph_captured_trips_match_oracle(schedule):
+    """Trips captured into HIP graphs of 8 (grids sized for every cell, kernels skipping
+    beyond their list counts) give the host-driven results."""
+    n = 4096
+    d = cells.generate_workload("drm19_128_mix", ncell=n)
+    ref = O.Mech("drm19").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    r = _mk("drm19", ode__rtol=1e-6, ode__atol=1e-10, mi__schedule=schedule, mi__graph_trips=8)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    assert st["nfail"] == 0 and st["graph_trips_used"] == 8 and st["ntrips"] % 8 == 0
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    assert (np.abs(rY.cpu().numpy() - ref["rY"]) / tol).max() < 10.0
+    assert np.abs(T.cpu().numpy() - ref["T"]).max() < 1e-2
+    assert np.mean(FC.cpu().numpy() == ref["FC"]) > 0.9
+
+
+def test_auto_schedule_tunes_once_per_box_size():
+    n = 4096
+    d = cells.generate_workload("lidryer_16", ncell=n)
+    ref = O.Mech("LiDryer").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    r = _mk("LiDryer", ode__rtol=1e-6, ode__atol=1e-10, mi__schedule="auto")
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    tunes = []
+    for call in range(2):
+        rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+        FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+        r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+        torch.cuda.synchronize()
+        st = r.stats()
+        assert st["nfail"] == 0
+        assert (np.abs(rY.cpu().numpy() - ref["rY"]) / tol).max() < 10.0
+        assert np.abs(T.cpu().numpy() - ref["T"]).max() < 1e-2
+        tunes.append(list(st["tune_ms"]))
+        best = int(np.argmin(st["tune_ms"]))
+        assert st["schedule_used"] == best % 2 and st["graph_trips_used"] == (8 if best >= 2 else 0)
+    r.close()
+    assert all(t > 0 for t in tunes[0]) and tunes[0] == tunes[1]   # tuned on the first call only
+    print(f"auto schedule, 4096 LiDryer cells: plain {tunes[0][0]:.2f} ms, queue {tunes[0][1]:.2f} ms, "
+          f"plain+graphs {tunes[0][2]:.2f} ms, queue+graphs {tunes[0][3]:.2f} ms")
