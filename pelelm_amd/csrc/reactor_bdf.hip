@@ -1779,10 +1779,10 @@ __global__ void null_kernel(const ReactArgs a, int nspec) {
   } while (0)
 
 // blocks for the cooperative kernels (two cells per block) handling up to n cells: a
-// multiple of 8, at most 32 resident blocks per CU of the 256
+// multiple of 8, at most 65536 (larger lists are strided over)
 inline unsigned coop_grid(long n) {
   const long want = ((n + 1) / 2 + 7) / 8 * 8;
-  return (unsigned)(want < 8192 ? (want > 0 ? want : 8) : 8192);
+  return (unsigned)(want < 65536 ? (want > 0 ? want : 8) : 65536);
 }
 
 template <class M>
