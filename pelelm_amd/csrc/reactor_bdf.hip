@@ -1385,11 +1385,17 @@ __device__ __forceinline__ long xcd_block() {
   return (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
 }
 
-template <class M, class Real>
+// STRIDE = false: one pair per block, the grid covering the list (host-driven trips);
+// STRIDE = true: a fixed grid striding over any count (captured graphs)
+template <class M, class Real, bool STRIDE>
 __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __restrict__ list,
                                                 const unsigned int* __restrict__ count) {
   const long n = (long)*count;
-  for (long blk = xcd_block(); blk * 2 < n; blk += gridDim.x) lu_coop_pair<M, Real>(a, list, n, blk);
+  if (STRIDE) {
+    for (long blk = xcd_block(); blk * 2 < n; blk += gridDim.x) lu_coop_pair<M, Real>(a, list, n, blk);
+  } else {
+    lu_coop_pair<M, Real>(a, list, n, xcd_block());
+  }
 }
 
 // true when the cell's Newton iteration proceeds this trip (not a singular set-up)
@@ -1539,11 +1545,15 @@ __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* _
   if (row) mat[MS::X + r] = x;
 }
 
-template <class M, class Real>
+template <class M, class Real, bool STRIDE>
 __global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int* __restrict__ list,
                                                    const unsigned int* __restrict__ count) {
   const long n = (long)*count;
-  for (long blk = xcd_block(); blk * 2 < n; blk += gridDim.x) solve_coop_pair<M, Real>(a, list, n, blk);
+  if (STRIDE) {
+    for (long blk = xcd_block(); blk * 2 < n; blk += gridDim.x) solve_coop_pair<M, Real>(a, list, n, blk);
+  } else {
+    solve_coop_pair<M, Real>(a, list, n, xcd_block());
+  }
 }
 
 // advance every active cell's state machine by this trip's evaluation
@@ -1778,30 +1788,42 @@ __global__ void null_kernel(const ReactArgs a, int nspec) {
     if (e_ != hipSuccess) return e_;    \
   } while (0)
 
-// blocks for the cooperative kernels (two cells per block) handling up to n cells: a
-// multiple of 8, at most 65536 (larger lists are strided over)
-inline unsigned coop_grid(long n) {
+// Blocks for the cooperative kernels (two cells per block), a multiple of 8: with
+// `exact` (the count n is known) enough for every pair, else a fixed 8192 blocks that
+// stride over whatever the list holds.
+inline unsigned coop_grid(long n, bool exact) {
+  if (!exact) return 8192;
   const long want = ((n + 1) / 2 + 7) / 8 * 8;
-  return (unsigned)(want < 65536 ? (want > 0 ? want : 8) : 65536);
+  return (unsigned)(want > 0 ? want : 8);
 }
 
+// exact = true: `n` is the list's length; false: an upper bound (graph capture)
 template <class M>
-hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* count, long n, hipStream_t stream) {
+hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* count, long n, hipStream_t stream,
+                     bool exact = true) {
   if (a.lu_lane) {
     hipLaunchKernelGGL(k_lu_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, list, count);
     return hipGetLastError();
   }
-  const dim3 grid(coop_grid(n));
+  const dim3 grid(coop_grid(n, exact));
   if (a.lu_mode == LU_FP32_IR) {
-    hipLaunchKernelGGL((k_lu_coop<M, float>), grid, dim3(64), 0, stream, a, list, count);
+    if (exact)
+      hipLaunchKernelGGL((k_lu_coop<M, float, false>), grid, dim3(64), 0, stream, a, list, count);
+    else
+      hipLaunchKernelGGL((k_lu_coop<M, float, true>), grid, dim3(64), 0, stream, a, list, count);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   // FP64 factorisations: every pending cell, or under LU_FP32_IR the cells that fell back
-  hipLaunchKernelGGL((k_lu_coop<M, double>), grid, dim3(64), 0, stream, a, list, count);
+  if (exact)
+    hipLaunchKernelGGL((k_lu_coop<M, double, false>), grid, dim3(64), 0, stream, a, list, count);
+  else
+    hipLaunchKernelGGL((k_lu_coop<M, double, true>), grid, dim3(64), 0, stream, a, list, count);
   return hipGetLastError();
 }
 
+// n bounds the solve list's length (the list is built by this trip's evaluation kernel),
+// so the grid strides
 template <class M>
 hipError_t launch_solve(const ReactArgs& a, long n, hipStream_t stream) {
   if (a.lu_lane) {
@@ -1809,13 +1831,13 @@ hipError_t launch_solve(const ReactArgs& a, long n, hipStream_t stream) {
                        &a.counts[6]);
     return hipGetLastError();
   }
-  const dim3 grid(coop_grid(n));
+  const dim3 grid(coop_grid(n, true));
   if (a.lu_mode == LU_FP32_IR) {
-    hipLaunchKernelGGL((k_solve_coop<M, float>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
+    hipLaunchKernelGGL((k_solve_coop<M, float, true>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((k_solve_coop<M, double>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
+  hipLaunchKernelGGL((k_solve_coop<M, double, true>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
   return hipGetLastError();
 }
 
@@ -1893,8 +1915,8 @@ hipError_t enqueue_fixed_trip(const ReactArgs& a, int cur, hipStream_t s) {
   PELE_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_setup<M>, dim3(nb), dim3(64), 0, s, a, a.slist[cur], &a.counts[2 + cur]);
   PELE_TRY(hipGetLastError());
-  PELE_TRY(launch_lu<M>(a, a.slist[cur], &a.counts[2 + cur], n, s));
-  PELE_TRY(launch_solve<M>(a, n, s));
+  PELE_TRY(launch_lu<M>(a, a.slist[cur], &a.counts[2 + cur], n, s, false));
+  PELE_TRY(launch_solve<M>(a, 16384, s));
   if (a.compact)
     hipLaunchKernelGGL(k_consume<M>, dim3(nb), dim3(64), 0, s, a, act, act_n, a.list[nxt], &a.counts[nxt],
                        a.slist[nxt], &a.counts[2 + nxt]);
