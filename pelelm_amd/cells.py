@@ -74,6 +74,7 @@ WORKLOADS = {
     "lidryer_16": Workload("LiDryer", "H2", 0.5, (0.5, 1.5), 0, 101, 16 ** 3, 1e-6),
     "drm19_64": Workload("drm19", "CH4", 2.0, (0.6, 1.2), 0, 102, 64 ** 3, 1e-6),
     "drm19_128_mix": Workload("drm19", "CH4", 2.0, (0.6, 1.2), 1, 104, 128 ** 3, 1e-5),
+    "dodecane53_64_mix": Workload("dodecane53", "NC12H26", 18.5, (0.6, 1.2), 1, 105, 64 ** 3, 1e-5),
 }
 
 

@@ -3,7 +3,7 @@
 SciPy's Radau IIA (order 5, scipy.integrate.solve_ivp) at rtol 1e-12 integrates the
 reactor right-hand side (the oracle's, T re-solved from the enthalpy) for seeded cells
 of every workload; the BDF restatements are checked against these end states.  Writes
-radau_<workload>.json.  Run: python tests/golden/make_radau_golden.py
+radau_<workload>.json.  Run: python tests/golden/make_radau_golden.py [workload ...]
 """
 import json
 import os
@@ -34,7 +34,10 @@ def advance(m, y0, rE0, dt):
 
 
 def main():
-    for wl, ncell in (("lidryer_16", 8), ("drm19_64", 8), ("drm19_128_mix", 8)):
+    only = set(sys.argv[1:])
+    for wl, ncell in (("lidryer_16", 8), ("drm19_64", 8), ("drm19_128_mix", 8), ("dodecane53_64_mix", 8)):
+        if only and wl not in only:
+            continue
         d = cells.generate_workload(wl, ncell=64)
         m = O.Mech(d["mech"])
         # a spread of progress variables: every 8th generated cell

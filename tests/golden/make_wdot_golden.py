@@ -3,7 +3,7 @@
 An mpmath (50-digit) evaluation of the rate law of Docs/source/manual/Model.rst:227-264
 straight from the mechanism YAML (not from the flat table the oracle reads), at seeded
 states spanning 300-2500 K and lean-to-rich compositions.  Writes wdot_<mech>.json.
-Run: python tests/golden/make_wdot_golden.py
+Run: python tests/golden/make_wdot_golden.py [mechanism ...]
 """
 import json
 import os
@@ -83,7 +83,10 @@ def wdot(mech, C, T):
 
 def main():
     out = {}
-    for name, nstate in (("LiDryer", 12), ("drm19", 12)):
+    only = set(sys.argv[1:])
+    for name, nstate in (("LiDryer", 12), ("drm19", 12), ("dodecane53", 12)):
+        if only and name not in only:
+            continue
         mech = M.load(name)
         rng = np.random.default_rng(2024)
         W = np.array(mech.W)

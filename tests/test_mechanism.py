@@ -4,8 +4,12 @@ The PMF profiles in the reference (Exec/RegTests/FlameSheet/drm19_pmf.dat,
 Exec/RegTests/EB_FlamePastCylinder/pmf_040_01Atm.dat) are the only numbers the reference
 holds for this path; their columns are mole fractions.  They pin molecular weights and
 the gas constant (every row obeys the ideal-gas law) and the enthalpies (a 1-D flame's
-far fields have equal mixture enthalpy).  Those two profiles are copied here as small
+far fields have equal mixture enthalpy).  Those profiles are copied here as small
 fixtures (first and last rows only) so the tests run without the reference tree.
+
+The 53-species n-dodecane mechanism keeps the species and order of
+Exec/RegTests/FlameSheet/dodecaneLu_pmf.dat:1; its profile pins the molecular weights and,
+through the far-field balance, the fuel's heat of formation against the core thermo.
 """
 import math
 import os
@@ -22,7 +26,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PMF_ORDER = {
     "drm19": "H2 H O O2 OH H2O HO2 CH2 CH2(S) CH3 CH4 CO CO2 HCO CH2O CH3O C2H4 C2H5 C2H6 N2 AR".split(),
     "LiDryer": "H2 O2 H2O H O OH HO2 H2O2 N2".split(),
+    "dodecane53": ("NC12H26 H O OH HO2 H2 H2O H2O2 O2 CH2 CH2D CH3 CH4 HCO CH2O CH3O CO CO2 C2H2 C2H3 C2H4 "
+                   "C2H5 C2H6 CH2CHO aC3H5 C3H6 nC3H7 C2H3CHO C4H7 C4H81 pC4H9 C5H9 C5H10 PXC5H11 C6H12 "
+                   "PXC6H13 C7H14 PXC7H15 C8H16 PXC8H17 C9H18 PXC9H19 C10H20 PXC10H21 PXC12H25 SXC12H25 "
+                   "S3XC12H25 C12H24 C12H25O2 C12OOH O2C12H24OOH OC12H23OOH N2").split(),
 }
+MECHS = ["drm19", "LiDryer", "dodecane53"]
 
 # heats of formation at 298.15 K [kJ/mol] (Active Thermochemical Tables / JANAF) and
 # the tolerance each is held to
@@ -30,21 +39,22 @@ DHF = {"H2": (0.0, 0.5), "O2": (0.0, 0.5), "N2": (0.0, 0.5), "AR": (0.0, 0.5), "
        "O": (249.2, 1.0), "OH": (38.0, 1.5), "H2O": (-241.8, 1.0), "HO2": (12.3, 1.5), "H2O2": (-135.9, 1.0),
        "CH4": (-74.6, 1.0), "CO": (-110.5, 1.0), "CO2": (-393.5, 1.0), "CH3": (146.6, 1.5), "CH2": (391.2, 2.0),
        "CH2(S)": (429.0, 2.0), "HCO": (41.6, 1.5), "CH2O": (-108.9, 1.5), "CH3O": (17.0, 6.0),
-       "C2H4": (52.4, 1.0), "C2H5": (120.0, 2.0), "C2H6": (-84.0, 1.0)}
+       "C2H4": (52.4, 1.0), "C2H5": (120.0, 2.0), "C2H6": (-84.0, 1.0), "CH2D": (429.0, 2.0),
+       "NC12H26": (-290.0, 3.0), "C2H2": (228.2, 2.0), "C2H3": (296.9, 4.0)}
 
 
 def fixture_rows(name):
     return np.load(os.path.join(HERE, "golden", f"pmf_endpoints_{name}.npz"))
 
 
-@pytest.mark.parametrize("name,nr", [("drm19", 84), ("LiDryer", 21)])
+@pytest.mark.parametrize("name,nr", [("drm19", 84), ("LiDryer", 21), ("dodecane53", 180)])
 def test_load_and_order(name, nr):
     m = M.load(name)
     assert m.names == PMF_ORDER[name]
     assert m.nreac == nr
 
 
-@pytest.mark.parametrize("name", ["drm19", "LiDryer"])
+@pytest.mark.parametrize("name", MECHS)
 def test_nasa_continuity_at_break(name):
     m = M.load(name)
     for sp in m.species:
@@ -60,16 +70,18 @@ def test_nasa_continuity_at_break(name):
         assert d.max() < 5e-5, (sp.name, d)
 
 
-@pytest.mark.parametrize("name", ["drm19", "LiDryer"])
+@pytest.mark.parametrize("name", MECHS)
 def test_heats_of_formation(name):
     m = M.load(name)
     for sp in m.species:
+        if sp.name not in DHF:     # synthesised dodecane53 thermo: no tabulated value to hold it to
+            continue
         ref, tol = DHF[sp.name]
         dhf = M.h_RT(sp, 298.15) * 298.15 * M.R_CGS * 1e-10   # erg/mol -> kJ/mol
         assert abs(dhf - ref) < tol, (sp.name, dhf, ref)
 
 
-@pytest.mark.parametrize("name", ["drm19", "LiDryer"])
+@pytest.mark.parametrize("name", MECHS)
 def test_cp_is_dh_dT(name):
     m = M.load(name)
     for sp in m.species:
@@ -91,7 +103,22 @@ def test_pmf_ideal_gas_density(name):
         assert abs(M.P_ATM * Wbar / (M.R_CGS * T) / rho - 1.0) < 1e-7
 
 
-@pytest.mark.parametrize("name,tolK", [("drm19", 1.0), ("LiDryer", 0.05)])
+def test_pmf_ideal_gas_density_dodecane53():
+    """The dodecane profile was written with a gas constant (or pressure) 1.56e-5 off the
+    CHEMKIN one, the same factor on every row; its two far fields (fuel/air and burnt
+    gas, of very different mean molecular weight) must give that same factor to 1e-9."""
+    m = M.load("dodecane53")
+    d = fixture_rows("dodecane53")
+    W = np.array(m.W)
+    f = []
+    for row in (d["first"], d["last"]):
+        T, rho, X = row[1], row[3], np.maximum(row[4:], 0.0)
+        f.append(M.P_ATM * np.dot(X, W) / X.sum() / (M.R_CGS * T) / rho)
+    assert abs(f[0] / f[1] - 1.0) < 1e-9
+    assert abs(f[0] - 1.0) < 2e-5
+
+
+@pytest.mark.parametrize("name,tolK", [("drm19", 1.0), ("LiDryer", 0.05), ("dodecane53", 0.5)])
 def test_pmf_far_field_enthalpy_balance(name, tolK):
     m = M.load(name)
     d = fixture_rows(name)
@@ -107,7 +134,7 @@ def test_pmf_far_field_enthalpy_balance(name, tolK):
 
 
 def test_reactions_element_balanced():
-    for name in ("drm19", "LiDryer"):
+    for name in MECHS:
         m = M.load(name)
         for r in m.reactions:
             for el in m.elements:

@@ -17,7 +17,7 @@ from oracle import oracle as O
 from pelelm_amd import cells
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-MECHS = ["LiDryer", "drm19"]
+MECHS = ["LiDryer", "drm19", "dodecane53"]
 
 
 def _random_state(m, rng, Tlo=300.0, Thi=2800.0):
@@ -65,7 +65,8 @@ def test_wdot_conserves_mass_and_elements(name):
         assert np.max(np.abs(m.ecomp.T.astype(float) @ w)) <= 1e-13 * np.abs(w).max() * m.ecomp.max()
 
 
-@pytest.mark.parametrize("name,fuel,o2", [("LiDryer", "H2", 0.5), ("drm19", "CH4", 2.0)])
+@pytest.mark.parametrize("name,fuel,o2", [("LiDryer", "H2", 0.5), ("drm19", "CH4", 2.0),
+                                         ("dodecane53", "NC12H26", 18.5)])
 @pytest.mark.parametrize("phi", [0.6, 1.0, 1.4])
 def test_detailed_balance_at_hp_equilibrium(name, fuel, o2, phi):
     """At the HP-equilibrium state (an independent element-potential solver) every
@@ -139,7 +140,7 @@ def test_bdf_error_scales_with_tolerance():
     assert errs[2] < 1e-6
 
 
-@pytest.mark.parametrize("wl", ["lidryer_16", "drm19_64", "drm19_128_mix"])
+@pytest.mark.parametrize("wl", ["lidryer_16", "drm19_64", "drm19_128_mix", "dodecane53_64_mix"])
 def test_bdf_matches_radau_fixture(wl):
     with open(os.path.join(HERE, "golden", f"radau_{wl}.json")) as f:
         g = json.load(f)
