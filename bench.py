@@ -50,7 +50,7 @@ def rhs_ops(mech: str):
     return fixed, per_iter, trans
 
 
-LANE_WORDS = 75   # 8-byte words of the integrator's lane state (drm19 and LiDryer)
+LANE_WORDS = 75   # 8-byte words of the integrator's lane state (independent of the mechanism)
 
 
 def rhs_bytes(N: int) -> int:
@@ -66,11 +66,11 @@ KIND_DESC = {"k_eval": "right-hand side (HY2T, rates, energy)", "k_setup": "anal
              "k_predict": "learned Newton predictor", "k_solve_coop": "Newton linear solves (substitution)"}
 
 
-def kernel_model(st, nspec):
+def kernel_model(st, nspec, mech):
     """Algorithmic work of each kernel kind over the call: (flops, hbm bytes)."""
     N, n = nspec, nspec + 1
-    fixed, per_iter, _ = rhs_ops("drm19")
-    with open(os.path.join(ROOT, "pelelm_amd", "mechanisms", "drm19_ops.json")) as f:
+    fixed, per_iter, _ = rhs_ops(mech)
+    with open(os.path.join(ROOT, "pelelm_amd", "mechanisms", f"{mech}_ops.json")) as f:
         jac_flops = json.load(f)["functions"]["jacobian"]["flops"]
     evals = st["kernel_cells"][0]
     nni, nst, nsetups, nje = st["nni"], st["nsteps"], st["nsetups"], st["nje"]
@@ -86,15 +86,15 @@ def kernel_model(st, nspec):
         # forward/back substitution with the factors (n^2 words) and B, X vectors
         "k_solve_coop": (nni * 2 * n * n, nni * 8 * (n * n + 2 * n)),
         "k_lu_coop": (nsetups * (2 * n ** 3 // 3 + 2 * n * n), nsetups * 8 * 2 * n * n),
-        "k_predict": (st["npred_used"] * 2 * (24 * 128 + 2 * 128 * 128 + 128 * 22), st["npred_used"] * 8 * 3 * n),
+        "k_predict": (st["npred_used"] * 2 * ((N + 3) * 128 + 2 * 128 * 128 + 128 * n), st["npred_used"] * 8 * 3 * n),
     }
 
 
-def measured_traffic(kernel: str, form: str):
+def measured_traffic(kernel: str, form: str, workload: str):
     """HBM bytes per launch of `kernel` measured by the byte counters (FETCH_SIZE and
     WRITE_SIZE, each scaled by what it reads for a copy of known size), from the
     committed summary for this workload; None when the summary has no such row."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_drm19_128_mix_{form}.md")
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_{form}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -107,7 +107,7 @@ def measured_traffic(kernel: str, form: str):
     return None
 
 
-def kernel_roofline(st, nspec, traffic_bytes=None, form="plain"):
+def kernel_roofline(st, nspec, mech, workload, traffic_bytes=None, form="plain"):
     """Roofline of the kernel with the largest device time: algorithmic work per launch
     over its mean launch time (HIP events on the reactor's stream), against FP64 peak
     (the vector and matrix FP64 peaks of MI355X coincide) or HBM peak for k_consume."""
@@ -116,7 +116,7 @@ def kernel_roofline(st, nspec, traffic_bytes=None, form="plain"):
     name = KINDS[dom]
     launches = max(1, st["kernel_launches"][dom])
     avg_ms = kms[dom] / launches
-    model = kernel_model(st, nspec)
+    model = kernel_model(st, nspec, mech)
     flops, byts = model[name]
     per_launch_flops, per_launch_bytes = flops / launches, byts / launches
     if name == "k_consume":
@@ -127,9 +127,9 @@ def kernel_roofline(st, nspec, traffic_bytes=None, form="plain"):
         achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
         line = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS}
-    line["traffic"] = traffic_bytes if traffic_bytes is not None else measured_traffic(name, form)
+    line["traffic"] = traffic_bytes if traffic_bytes is not None else measured_traffic(name, form, workload)
     line.update({
-        "kernel": f"{name}<mech_drm19>: {KIND_DESC[name]}",
+        "kernel": f"{name}<mech_{mech}>: {KIND_DESC[name]}",
         "avg_launch_ms": avg_ms, "launches": launches,
         "algorithmic_flops_per_launch": per_launch_flops, "algorithmic_bytes_per_launch": per_launch_bytes,
         "kernel_ms": {KINDS[i]: kms[i] for i in range(len(KINDS))},
@@ -142,11 +142,11 @@ def kernel_roofline(st, nspec, traffic_bytes=None, form="plain"):
     return line
 
 
-def run_form(form, dev_in, ncell, nspec, dt, steps, warmup, profile):
+def run_form(form, mech, dev_in, ncell, nspec, dt, steps, warmup, profile):
     import torch
     from pelelm_amd import reactor
     r = reactor.ReactorBase.create("ReactorCvode")
-    r.set("mi.mechanism", "drm19")
+    r.set("mi.mechanism", mech)
     r.set("ode.rtol", 1e-6)
     r.set("ode.atol", 1e-10)
     r.set("cvode.max_order", 4)
@@ -229,7 +229,7 @@ def main(argv=None):
 
     forms = {}
     for form in args.forms.split(","):
-        forms[form] = run_form(form, dev_in, ncell, nspec, d["dt"], args.steps, args.warmup,
+        forms[form] = run_form(form, d["mech"], dev_in, ncell, nspec, d["dt"], args.steps, args.warmup,
                                profile=not args.no_profile)
         log(f"{form}: {forms[form]['cells_per_s']:.4g} cells/s, {forms[form]['ms_per_step']:.1f} ms/step")
     best = max(forms, key=lambda f: forms[f]["cells_per_s"])
@@ -238,7 +238,7 @@ def main(argv=None):
 
     roofline = None
     if not args.no_profile:
-        roofline = kernel_roofline(st, nspec, traffic_bytes=args.traffic_bytes, form=best)
+        roofline = kernel_roofline(st, nspec, d["mech"], args.workload, traffic_bytes=args.traffic_bytes, form=best)
 
     cpu = None
     if not args.no_cpu:
@@ -268,13 +268,16 @@ def main(argv=None):
                "gpu_vs_cpu_max_abs_dT_K": float(np.abs(gT - ref["T"]).max()),
                "gpu_vs_cpu_max_rel_err_over_tol": float((np.abs(gY - ref["rY"]) / tol).max())}
 
+    wl = cells.WORKLOADS[args.workload]
+    side = round(ncell ** (1.0 / 3.0))
     line = {
         "metric": METRIC, "value": bf["cells_per_s"], "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": bf["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic: seeded Philox cells, fresh CH4/air (phi U[0.6,1.2]) blended with HP-equilibrium products",
-        "config": {"workload": args.workload, "mechanism": "drm19", "ncell": ncell, "box": "128^3" if ncell == 128 ** 3
-                   else f"{ncell} cells", "dt": d["dt"], "rtol": 1e-6, "atol": 1e-10, "max_order": 4,
+        "data": f"synthetic: seeded Philox cells, fresh {wl.fuel}/air (phi U[{wl.phi[0]},{wl.phi[1]}]) blended "
+                f"with HP-equilibrium products",
+        "config": {"workload": args.workload, "mechanism": d["mech"], "ncell": ncell,
+                   "box": f"{side}^3" if side ** 3 == ncell else f"{ncell} cells", "dt": d["dt"], "rtol": 1e-6, "atol": 1e-10, "max_order": 4,
                    "form": best, "integrator": "BDF (CVODE algorithm), dense analytic-Jacobian LU, FP64"},
         "forms": {f: {"cells_per_s": v["cells_per_s"], "ms_per_step": v["ms_per_step"], "nfail": v["nfail"],
                       "steps_per_cell": v["stats"]["nsteps"] / max(1, v["stats"]["ncells"]),

@@ -42,6 +42,7 @@
 
 #include "mech/LiDryer.inc"
 #include "mech/drm19.inc"
+#include "mech/dodecane53.inc"
 
 namespace pelelm {
 namespace {
@@ -101,7 +102,9 @@ struct MatSlots {
   static constexpr int IRS = X + N1;                  // refinement sweeps of the last solve
   static constexpr int IRF = IRS + 1;                 // 1: refinement did not converge
   static constexpr int PERM = IRF + 1;                // N1 ints packed two per word
-  static constexpr int USED = PERM + (N1 + 1) / 2;
+  static constexpr int R = PERM + (N1 + 1) / 2;       // refinement residual (one-lane solves)
+  static constexpr int D = R + N1;                    // refinement correction (one-lane solves)
+  static constexpr int USED = D + N1;
   static constexpr int STRIDE = (USED + 15) / 16 * 16; // whole 128-byte lines per cell
 };
 
@@ -288,31 +291,28 @@ __device__ __noinline__ int lu_factor(const CM& mat, const double gamma) {
   return sing;
 }
 
-// one lane solves M x = b (b in registers) with the factors of the matrix region, FP64
-// arithmetic on factors stored in Real.  P M = L U, row i of the factors holding row
-// PERM[i] of M.
+// one lane solves M x = b with the factors of the matrix region, FP64 arithmetic on
+// factors stored in Real.  P M = L U, row i of the factors holding row PERM[i] of M.
+// b and x are vectors of the cell's matrix region (word offsets src, dst; distinct),
+// so the solve holds no N1-long array in registers.
 template <int N1, class Real = double>
-__device__ __forceinline__ void lu_solve(const CM& mat, const double* __restrict__ b, double* __restrict__ x) {
+__device__ __forceinline__ void lu_solve(const CM& mat, const int src, const int dst) {
   using MS = MatSlots<N1>;
   using A = LUAcc<N1, Real>;
-#pragma unroll
-  for (int i = 0; i < N1; ++i) {
-    const int pi = mat.perm(MS::PERM, i);
-    double v = 0.0;
-#pragma unroll
-    for (int j = 0; j < N1; ++j) v = (j == pi) ? b[j] : v;   // register gather of b[PERM[i]]
-    x[i] = v;
+#pragma unroll 1
+  for (int i = 0; i < N1; ++i) mat[dst + i] = mat[src + mat.perm(MS::PERM, i)];
+#pragma unroll 1
+  for (int k = 0; k < N1; ++k) {
+    const double xk = mat[dst + k];
+#pragma unroll 1
+    for (int i = k + 1; i < N1; ++i) mat[dst + i] -= (double)A::at(mat, i * N1 + k) * xk;
   }
-#pragma unroll
-  for (int k = 0; k < N1; ++k)
-#pragma unroll
-    for (int i = k + 1; i < N1; ++i) x[i] -= (double)A::at(mat, i * N1 + k) * x[k];
-#pragma unroll
+#pragma unroll 1
   for (int k = N1 - 1; k >= 0; --k) {
-    double s = x[k];
-#pragma unroll
-    for (int j = k + 1; j < N1; ++j) s -= (double)A::at(mat, k * N1 + j) * x[j];
-    x[k] = s / (double)A::at(mat, k * N1 + k);
+    double s = mat[dst + k];
+#pragma unroll 1
+    for (int j = k + 1; j < N1; ++j) s -= (double)A::at(mat, k * N1 + j) * mat[dst + j];
+    mat[dst + k] = s / (double)A::at(mat, k * N1 + k);
   }
 }
 
@@ -1287,9 +1287,18 @@ __global__ void __launch_bounds__(64) k_lu_lane(const ReactArgs a, const int* __
 }
 
 
-// Cooperative LU of M = I - gamma J in Real arithmetic: 32 lanes per cell (row r of M
-// in lane r), two cells per wave.  Column k's pivot is the largest |m_rk| over the rows
-// not yet pivoted (a 32-lane max-reduction); its row is broadcast through LDS and every
+// Lanes per cell of the cooperative LU and solves: a 32-lane half-wave (two cells per
+// wave) up to 32 unknowns, the whole wave (one cell) up to 64.
+template <int N1>
+struct Coop {
+  static_assert(N1 <= 64, "the cooperative kernels hold one matrix row per lane of a wave");
+  static constexpr int G = N1 <= 32 ? 32 : 64;
+  static constexpr int CPW = 64 / G;
+};
+
+// Cooperative LU of M = I - gamma J in Real arithmetic: G lanes per cell (row r of M
+// in lane r), 64 / G cells per wave.  Column k's pivot is the largest |m_rk| over the rows
+// not yet pivoted (a G-lane max-reduction); its row is broadcast through LDS and every
 // other remaining row is eliminated in registers.  Row r ends as row `step` of the
 // factors, where `step` is the column at which it was chosen (PERM[step] = r), which
 // is partial pivoting with the interchanges kept as a permutation.  A launch handles
@@ -1299,14 +1308,14 @@ template <class M, class Real>
 __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __restrict__ list, const long n,
                                              const long blk) {
   constexpr int N1 = M::N1;
-  static_assert(N1 <= 32, "the cooperative LU holds one row per lane of a 32-lane half-wave");
+  constexpr int G = Coop<N1>::G, CPW = Coop<N1>::CPW;
   using S = Slots<N1>;
   using A = LUAcc<N1, Real>;
   constexpr bool IS32 = sizeof(Real) == 4;
-  __shared__ Real prow[2][N1];
+  __shared__ Real prow[CPW][N1];
   const int lane = threadIdx.x;
-  const int half = lane >> 5, r = lane & 31;
-  const long idx = blk * 2 + half;
+  const int half = lane / G, r = lane % G;
+  const long idx = blk * CPW + half;
   long c = -1;
   bool work = false;
   double gamma = 0.0;
@@ -1336,9 +1345,9 @@ __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __re
     if (v != v) v = (Real)HUGE_VAL;   // a NaN row still takes a definite place in the order
     int who = r;
 #pragma unroll
-    for (int off = 16; off > 0; off >>= 1) {
-      const Real ov = __shfl_xor(v, off, 32);
-      const int ow = __shfl_xor(who, off, 32);
+    for (int off = G / 2; off > 0; off >>= 1) {
+      const Real ov = __shfl_xor(v, off, G);
+      const int ow = __shfl_xor(who, off, G);
       if (ov > v || (ov == v && ow < who)) {
         v = ov;
         who = ow;
@@ -1350,7 +1359,7 @@ __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __re
       for (int j = 0; j < N1; ++j) prow[half][j] = m[j];
     }
     __syncthreads();
-    if (!done && r != who && v > (Real)0) {   // v: this column's pivot magnitude, equal in all 32 lanes
+    if (!done && r != who && v > (Real)0) {   // v: this column's pivot magnitude, equal in all G lanes
       const Real inv = (Real)1 / prow[half][k];
       const Real l = m[k] * inv;
       m[k] = l;
@@ -1373,7 +1382,7 @@ __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __re
     mat.perm(MatSlots<N1>::PERM, my_step) = r;
     if (r == 0) w[S::SING] = (double)sing;
   }
-  __syncthreads();   // prow is reused by the next pair of this block
+  __syncthreads();   // prow is reused by the next group of this block
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin: each XCD gets contiguous ranges of
@@ -1385,14 +1394,14 @@ __device__ __forceinline__ long xcd_block() {
   return (long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
 }
 
-// STRIDE = false: one pair per block, the grid covering the list (host-driven trips);
-// STRIDE = true: a fixed grid striding over any count (captured graphs)
+// STRIDE = false: one wave's cells per block, the grid covering the list (host-driven
+// trips); STRIDE = true: a fixed grid striding over any count (captured graphs)
 template <class M, class Real, bool STRIDE>
 __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __restrict__ list,
                                                 const unsigned int* __restrict__ count) {
   const long n = (long)*count;
   if (STRIDE) {
-    for (long blk = xcd_block(); blk * 2 < n; blk += gridDim.x) lu_coop_pair<M, Real>(a, list, n, blk);
+    for (long blk = xcd_block(); blk * Coop<M::N1>::CPW < n; blk += gridDim.x) lu_coop_pair<M, Real>(a, list, n, blk);
   } else {
     lu_coop_pair<M, Real>(a, list, n, xcd_block());
   }
@@ -1421,61 +1430,56 @@ __global__ void __launch_bounds__(64) k_solve_lane(const ReactArgs a, const int*
   const WS w = cell_ws<M>(a.ws, c);
   if (!solve_due<M>(w)) return;
   const CM mat = cell_mat<M>(a.mat, c);
-  double b[N1], x[N1];
-#pragma unroll
-  for (int i = 0; i < N1; ++i) b[i] = mat[MS::B + i];
   const bool fp32 = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
   if (!fp32) {
-    lu_solve<N1, double>(mat, b, x);
-  } else {
-    lu_solve<N1, float>(mat, b, x);
-    const double gp = LANE_FIELD(M, w, gammap);
-    int sweeps = 0;
-    bool ok = (a.ir_sweeps == 0);
-#pragma unroll 1
-    for (; sweeps < a.ir_sweeps && !ok;) {
-      ++sweeps;
-      double r[N1], d[N1];
-#pragma unroll 1
-      for (int i = 0; i < N1; ++i) {
-        double v = b[i] - x[i];
-#pragma unroll
-        for (int j = 0; j < N1; ++j) v += gp * mat[MS::JS + i * N1 + j] * x[j];
-        r[i] = v;
-      }
-      lu_solve<N1, float>(mat, r, d);
-      double nrm = 0.0;
-#pragma unroll
-      for (int i = 0; i < N1; ++i) {
-        x[i] += d[i];
-        const double q = d[i] * w[S::EWT + i];
-        nrm += q * q;
-      }
-      ok = (sqrt(nrm / N1) <= IR_TOL);   // false for a NaN norm
-    }
-    mat[MS::IRS] = (double)sweeps;
-    mat[MS::IRF] = ok ? 0.0 : 1.0;
+    lu_solve<N1, double>(mat, MS::B, MS::X);
+    return;
   }
-#pragma unroll
-  for (int i = 0; i < N1; ++i) mat[MS::X + i] = x[i];
+  lu_solve<N1, float>(mat, MS::B, MS::X);
+  const double gp = LANE_FIELD(M, w, gammap);
+  int sweeps = 0;
+  bool ok = (a.ir_sweeps == 0);
+#pragma unroll 1
+  for (; sweeps < a.ir_sweeps && !ok;) {
+    ++sweeps;
+#pragma unroll 1
+    for (int i = 0; i < N1; ++i) {
+      double v = mat[MS::B + i] - mat[MS::X + i];
+#pragma unroll 1
+      for (int j = 0; j < N1; ++j) v += gp * mat[MS::JS + i * N1 + j] * mat[MS::X + j];
+      mat[MS::R + i] = v;
+    }
+    lu_solve<N1, float>(mat, MS::R, MS::D);
+    double nrm = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < N1; ++i) {
+      const double d = mat[MS::D + i];
+      mat[MS::X + i] += d;
+      const double q = d * w[S::EWT + i];
+      nrm += q * q;
+    }
+    ok = (sqrt(nrm / N1) <= IR_TOL);   // false for a NaN norm
+  }
+  mat[MS::IRS] = (double)sweeps;
+  mat[MS::IRF] = ok ? 0.0 : 1.0;
 }
 
-// The same solves with 32 lanes per cell (row r of the factors in lane r), two cells
+// The same solves with G lanes per cell (row r of the factors in lane r), 64 / G cells
 // per wave: forward and back substitution broadcast each finished component with a
-// 32-lane shuffle; a refinement residual gathers x the same way.  A launch handles the
+// G-lane shuffle; a refinement residual gathers x the same way.  A launch handles the
 // cells whose factor precision is Real.  No barriers: the two halves of a wave may
 // take different numbers of refinement sweeps.
 template <class M, class Real>
 __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* __restrict__ list, const long n,
                                                 const long blk) {
   constexpr int N1 = M::N1;
-  static_assert(N1 <= 32, "one row per lane of a 32-lane half-wave");
+  constexpr int G = Coop<N1>::G, CPW = Coop<N1>::CPW;
   using S = Slots<N1>;
   using MS = MatSlots<N1>;
   using A = LUAcc<N1, Real>;
   constexpr bool IS32 = sizeof(Real) == 4;
-  const int half = threadIdx.x >> 5, r = threadIdx.x & 31;
-  const long idx = blk * 2 + half;
+  const int half = threadIdx.x / G, r = threadIdx.x % G;
+  const long idx = blk * CPW + half;
   bool work = false;
   long c = 0;
   if (idx < n) {
@@ -1500,20 +1504,20 @@ __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* _
   auto subst = [&](double xv) {
 #pragma unroll
     for (int k = 0; k < N1 - 1; ++k) {
-      const double xk = __shfl(xv, k, 32);
+      const double xk = __shfl(xv, k, G);
       if (r > k) xv -= (double)lrow[k] * xk;
     }
 #pragma unroll
     for (int k = N1 - 1; k >= 0; --k) {
       if (r == k) xv /= (double)lrow[k];
-      const double xk = __shfl(xv, k, 32);
+      const double xk = __shfl(xv, k, G);
       if (r < k) xv -= (double)lrow[k] * xk;
     }
     return row ? xv : 0.0;
   };
   double x = subst(row ? mat[MS::B + pr] : 0.0);
   if (IS32) {
-    // refinement of the FP32 solve against the FP64 matrix (uniform per half-wave)
+    // refinement of the FP32 solve against the FP64 matrix (uniform per cell's lanes)
     const WS w = cell_ws<M>(a.ws, c);
     const double gp = work ? LANE_FIELD(M, w, gammap) : 0.0;
     const double bi = row ? mat[MS::B + r] : 0.0;
@@ -1526,15 +1530,15 @@ __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* _
       double res = bi - x;
 #pragma unroll
       for (int j = 0; j < N1; ++j) {
-        const double xj = __shfl(x, j, 32);
+        const double xj = __shfl(x, j, G);
         if (row) res += gp * mat[MS::JS + r * N1 + j] * xj;
       }
-      const double d = subst(__shfl(res, pr, 32));
+      const double d = subst(__shfl(res, pr, G));
       x += d;
       double q = d * ewt;
       q *= q;
 #pragma unroll
-      for (int off = 16; off > 0; off >>= 1) q += __shfl_xor(q, off, 32);
+      for (int off = G / 2; off > 0; off >>= 1) q += __shfl_xor(q, off, G);
       ok = (sqrt(q / N1) <= IR_TOL);
     }
     if (work && r == 0) {
@@ -1550,7 +1554,7 @@ __global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int*
                                                    const unsigned int* __restrict__ count) {
   const long n = (long)*count;
   if (STRIDE) {
-    for (long blk = xcd_block(); blk * 2 < n; blk += gridDim.x) solve_coop_pair<M, Real>(a, list, n, blk);
+    for (long blk = xcd_block(); blk * Coop<M::N1>::CPW < n; blk += gridDim.x) solve_coop_pair<M, Real>(a, list, n, blk);
   } else {
     solve_coop_pair<M, Real>(a, list, n, xcd_block());
   }
@@ -1788,12 +1792,14 @@ __global__ void null_kernel(const ReactArgs a, int nspec) {
     if (e_ != hipSuccess) return e_;    \
   } while (0)
 
-// Blocks for the cooperative kernels (two cells per block), a multiple of 8: with
-// `exact` (the count n is known) enough for every pair, else a fixed 8192 blocks that
+// Blocks for the cooperative kernels (CPW cells per block), a multiple of 8: with
+// `exact` (the count n is known) enough for every cell, else a fixed 8192 blocks that
 // stride over whatever the list holds.
+template <int N1>
 inline unsigned coop_grid(long n, bool exact) {
+  constexpr int CPW = Coop<N1>::CPW;
   if (!exact) return 8192;
-  const long want = ((n + 1) / 2 + 7) / 8 * 8;
+  const long want = ((n + CPW - 1) / CPW + 7) / 8 * 8;
   return (unsigned)(want > 0 ? want : 8);
 }
 
@@ -1805,7 +1811,7 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
     hipLaunchKernelGGL(k_lu_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, list, count);
     return hipGetLastError();
   }
-  const dim3 grid(coop_grid(n, exact));
+  const dim3 grid(coop_grid<M::N1>(n, exact));
   if (a.lu_mode == LU_FP32_IR) {
     if (exact)
       hipLaunchKernelGGL((k_lu_coop<M, float, false>), grid, dim3(64), 0, stream, a, list, count);
@@ -1831,7 +1837,7 @@ hipError_t launch_solve(const ReactArgs& a, long n, hipStream_t stream) {
                        &a.counts[6]);
     return hipGetLastError();
   }
-  const dim3 grid(coop_grid(n, true));
+  const dim3 grid(coop_grid<M::N1>(n, true));
   if (a.lu_mode == LU_FP32_IR) {
     hipLaunchKernelGGL((k_solve_coop<M, float, true>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
     const hipError_t e = hipGetLastError();
@@ -2100,6 +2106,7 @@ int mech_nspec(int mech) {
   switch (mech) {
     case MECH_LIDRYER: return mech_LiDryer::NS;
     case MECH_DRM19: return mech_drm19::NS;
+    case MECH_DODECANE53: return mech_dodecane53::NS;
     default: return -1;
   }
 }
@@ -2107,6 +2114,7 @@ const char* mech_name(int mech) {
   switch (mech) {
     case MECH_LIDRYER: return mech_LiDryer::NAME;
     case MECH_DRM19: return mech_drm19::NAME;
+    case MECH_DODECANE53: return mech_dodecane53::NAME;
     default: return "";
   }
 }
@@ -2123,6 +2131,7 @@ long ws_slots(int mech) {
   switch (mech) {
     case MECH_LIDRYER: return Slots<mech_LiDryer::N1>::TOTAL;
     case MECH_DRM19: return Slots<mech_drm19::N1>::TOTAL;
+    case MECH_DODECANE53: return Slots<mech_dodecane53::N1>::TOTAL;
     default: return 0;
   }
 }
@@ -2130,6 +2139,7 @@ long iws_slots(int mech) {
   switch (mech) {
     case MECH_LIDRYER: return Slots<mech_LiDryer::N1>::ITOTAL;
     case MECH_DRM19: return Slots<mech_drm19::N1>::ITOTAL;
+    case MECH_DODECANE53: return Slots<mech_dodecane53::N1>::ITOTAL;
     default: return 0;
   }
 }
@@ -2138,6 +2148,7 @@ long mat_stride(int mech) {
   switch (mech) {
     case MECH_LIDRYER: return MatSlots<mech_LiDryer::N1>::STRIDE;
     case MECH_DRM19: return MatSlots<mech_drm19::N1>::STRIDE;
+    case MECH_DODECANE53: return MatSlots<mech_dodecane53::N1>::STRIDE;
     default: return 0;
   }
 }
@@ -2147,6 +2158,7 @@ hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_
   switch (mech) {
     case MECH_LIDRYER: return run_react_t<mech_LiDryer>(a, stream, max_trips, trips_out, prof, pred, plan);
     case MECH_DRM19: return run_react_t<mech_drm19>(a, stream, max_trips, trips_out, prof, pred, plan);
+    case MECH_DODECANE53: return run_react_t<mech_dodecane53>(a, stream, max_trips, trips_out, prof, pred, plan);
     default: return hipErrorInvalidValue;
   }
 }
@@ -2156,6 +2168,7 @@ hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T,
   switch (mech) {
     case MECH_LIDRYER: return launch_rates_t<mech_LiDryer>(rY, rH, T, mask, rYdot, ncell, stream);
     case MECH_DRM19: return launch_rates_t<mech_drm19>(rY, rH, T, mask, rYdot, ncell, stream);
+    case MECH_DODECANE53: return launch_rates_t<mech_dodecane53>(rY, rH, T, mask, rYdot, ncell, stream);
     default: return hipErrorInvalidValue;
   }
 }

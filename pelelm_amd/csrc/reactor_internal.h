@@ -71,7 +71,7 @@ enum CounterIndex : int {
 };
 
 // Mechanism ids (order of the dispatch table in reactor_bdf.hip).
-enum MechId : int { MECH_LIDRYER = 0, MECH_DRM19 = 1, NMECH = 2 };
+enum MechId : int { MECH_LIDRYER = 0, MECH_DRM19 = 1, MECH_DODECANE53 = 2, NMECH = 3 };
 
 int mech_nspec(int mech);
 const char* mech_name(int mech);

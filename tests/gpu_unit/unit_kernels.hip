@@ -76,10 +76,9 @@ __global__ void k_jac_lu(const double* y, const double* T, const double* fT, con
     jacobian<M>(yy, T[c], fT[c], w);
     for (int e = 0; e < N1 * N1; ++e) mat[MatSlots<N1>::JS + e] = w[S::JS + e];
     sg = lu_factor<N1, double>(mat, gamma[c]);
-    double bb[N1], xx[N1];
-    for (int i = 0; i < N1; ++i) bb[i] = b[c * N1 + i];
-    lu_solve<N1>(mat, bb, xx);
-    for (int i = 0; i < N1; ++i) x[c * N1 + i] = xx[i];
+    for (int i = 0; i < N1; ++i) mat[MatSlots<N1>::B + i] = b[c * N1 + i];
+    lu_solve<N1>(mat, MatSlots<N1>::B, MatSlots<N1>::X);
+    for (int i = 0; i < N1; ++i) x[c * N1 + i] = mat[MatSlots<N1>::X + i];
     for (int e = 0; e < N1 * N1; ++e) J[c * N1 * N1 + e] = w[S::JS + e];
   }
   sing[c] = sg;
@@ -419,6 +418,7 @@ extern "C" int unit_lu(const char* mech, int coop, int fp32, int sweeps, long n,
                        int* perm, double* ir) {
   const int m = mech_lookup(mech);
   if (m == MECH_DRM19) return unit_lu_t<mech_drm19>(coop, fp32, sweeps, n, J, gamma, pend, b, ewt, x, sing, lu, perm, ir);
+  if (m == MECH_DODECANE53) return unit_lu_t<mech_dodecane53>(coop, fp32, sweeps, n, J, gamma, pend, b, ewt, x, sing, lu, perm, ir);
   if (m == MECH_LIDRYER)
     return unit_lu_t<mech_LiDryer>(coop, fp32, sweeps, n, J, gamma, pend, b, ewt, x, sing, lu, perm, ir);
   return -1;
@@ -428,6 +428,7 @@ extern "C" int unit_rhs(const char* mech, long n, const double* y, const double*
                         const double* fH, const double* fy, double* f, double* T) {
   const int m = mech_lookup(mech);
   if (m == MECH_DRM19) return unit_rhs_t<mech_drm19>(n, y, t, rE0, fH, fy, f, T);
+  if (m == MECH_DODECANE53) return unit_rhs_t<mech_dodecane53>(n, y, t, rE0, fH, fy, f, T);
   if (m == MECH_LIDRYER) return unit_rhs_t<mech_LiDryer>(n, y, t, rE0, fH, fy, f, T);
   return -1;
 }
@@ -435,12 +436,14 @@ extern "C" int unit_jac_lu(const char* mech, long n, const double* y, const doub
                            const double* gamma, const double* b, double* J, double* x, double* keep, int* sing) {
   const int m = mech_lookup(mech);
   if (m == MECH_DRM19) return unit_jac_lu_t<mech_drm19>(n, y, T, fT, gamma, b, J, x, keep, sing);
+  if (m == MECH_DODECANE53) return unit_jac_lu_t<mech_dodecane53>(n, y, T, fT, gamma, b, J, x, keep, sing);
   if (m == MECH_LIDRYER) return unit_jac_lu_t<mech_LiDryer>(n, y, T, fT, gamma, b, J, x, keep, sing);
   return -1;
 }
 extern "C" int unit_coeffs(const char* mech, long n, const double* scal, const double* zn, double* out) {
   const int m = mech_lookup(mech);
   if (m == MECH_DRM19) return unit_coeffs_t<mech_drm19>(n, scal, zn, out);
+  if (m == MECH_DODECANE53) return unit_coeffs_t<mech_dodecane53>(n, scal, zn, out);
   if (m == MECH_LIDRYER) return unit_coeffs_t<mech_LiDryer>(n, scal, zn, out);
   return -1;
 }
@@ -448,6 +451,7 @@ extern "C" int unit_stepctl(const char* mech, long n, const double* scal, const 
                             const double* ewt, double* out, const double* px, const double* py, double* pz) {
   const int m = mech_lookup(mech);
   if (m == MECH_DRM19) return unit_stepctl_t<mech_drm19>(n, scal, zn, acor, ewt, out, px, py, pz);
+  if (m == MECH_DODECANE53) return unit_stepctl_t<mech_dodecane53>(n, scal, zn, acor, ewt, out, px, py, pz);
   if (m == MECH_LIDRYER) return unit_stepctl_t<mech_LiDryer>(n, scal, zn, acor, ewt, out, px, py, pz);
   return -1;
 }

@@ -31,7 +31,8 @@ def _dev(a, dtype=torch.float64):
     return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
 
 
-@pytest.mark.parametrize("wl,mech", [("lidryer_16", "LiDryer"), ("drm19_64", "drm19")])
+@pytest.mark.parametrize("wl,mech", [("lidryer_16", "LiDryer"), ("drm19_64", "drm19"),
+                                     ("dodecane53_64_mix", "dodecane53")])
 def test_rates_match_oracle(wl, mech):
     d = cells.generate_workload(wl, ncell=2048)
     m = O.Mech(mech)
@@ -54,7 +55,7 @@ def test_rates_match_oracle(wl, mech):
 
 
 @pytest.mark.parametrize("wl,mech,n", [("lidryer_16", "LiDryer", 4096), ("drm19_64", "drm19", 4096),
-                                       ("drm19_128_mix", "drm19", 4096)])
+                                       ("drm19_128_mix", "drm19", 4096), ("dodecane53_64_mix", "dodecane53", 2048)])
 @pytest.mark.parametrize("schedule", ["plain", "queue"])
 def test_react_matches_oracle(wl, mech, n, schedule):
     d = cells.generate_workload(wl, ncell=n)
@@ -82,7 +83,8 @@ def test_react_matches_oracle(wl, mech, n, schedule):
     np.testing.assert_allclose(gY.sum(0), d["rY"].sum(0), rtol=1e-12)
 
 
-@pytest.mark.parametrize("wl,mech", [("drm19_64", "drm19"), ("lidryer_16", "LiDryer")])
+@pytest.mark.parametrize("wl,mech", [("drm19_64", "drm19"), ("lidryer_16", "LiDryer"),
+                                     ("dodecane53_64_mix", "dodecane53")])
 def test_fp32_ir_matches_fp64_and_tight_oracle(wl, mech):
     """Mixed precision (FP32 factors, up to 3 FP64-residual refinement sweeps per solve)
     against the FP64 oracle at the same rtol and against the tight oracle (rtol 1e-12):
@@ -191,14 +193,15 @@ def test_react_box_on_array4_views_with_ghost_cells():
     assert np.abs(got[N + 1] - ref["T"]).max() < 1e-2
 
 
+@pytest.mark.parametrize("wl,mech", [("drm19_128_mix", "drm19"), ("dodecane53_64_mix", "dodecane53")])
 @pytest.mark.parametrize("schedule", ["plain", "queue"])
-def test_graph_captured_trips_match_oracle(schedule):
+def test_graph_captured_trips_match_oracle(schedule, wl, mech):
     """Trips captured into HIP graphs of 8 (grids sized for every cell, kernels skipping
     beyond their list counts) give the host-driven results."""
     n = 4096
-    d = cells.generate_workload("drm19_128_mix", ncell=n)
-    ref = O.Mech("drm19").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
-    r = _mk("drm19", ode__rtol=1e-6, ode__atol=1e-10, mi__schedule=schedule, mi__graph_trips=8)
+    d = cells.generate_workload(wl, ncell=n)
+    ref = O.Mech(mech).react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    r = _mk(mech, ode__rtol=1e-6, ode__atol=1e-10, mi__schedule=schedule, mi__graph_trips=8)
     rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
     FC = torch.zeros(n, dtype=torch.float64, device=DEV)
     r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)

@@ -38,7 +38,7 @@ def emu():
 
 
 @pytest.mark.parametrize("wl,mech,n", [("lidryer_16", "LiDryer", 256), ("drm19_64", "drm19", 256),
-                                       ("drm19_128_mix", "drm19", 256)])
+                                       ("drm19_128_mix", "drm19", 256), ("dodecane53_64_mix", "dodecane53", 128)])
 @pytest.mark.parametrize("compact", [1, 0])
 def test_emulated_kernels_match_oracle(emu, wl, mech, n, compact):
     d = cells.generate_workload(wl, ncell=n)
