@@ -24,6 +24,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstddef>
+#include <utility>
 
 #include "reactor_internal.h"
 #include "mlp_internal.h"
@@ -187,41 +188,44 @@ __device__ __forceinline__ void rhs(const double* __restrict__ y, const double t
   f[N] = (fH - hw * M::RU * T) / (rhocp * M::RU);
 }
 
-// analytic Jacobian of the right-hand side at (y, T) with T held fixed, into ws[JS]
+// analytic Jacobian of the right-hand side at (y, T) with T held fixed, into ws[JS].
+// Every entry is written once (no accumulation in memory); the T row's sum over a
+// column is formed as that column's entries are produced.
 template <class M>
 __device__ __noinline__ void jacobian(const double* __restrict__ y, const double T, const double fT, const WS& ws) {
   constexpr int N = M::NS, N1 = M::N1;
   using S = Slots<N1>;
-#pragma unroll 1
-  for (int e = 0; e < N1 * N1; ++e) ws[S::JS + e] = 0.0;
   const double logT = log(T), invT = 1.0 / T;
   double C[N], g[N], hRT[N], cpR[N], wd[N], dwdT[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) C[k] = y[k] * M::imw(k);
   M::gibbs(T, logT, g);
   M::thermo_cph(T, cpR, hRT);
+  double rhocp = 0.0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) rhocp += y[k] * cpR[k] * M::imw(k);
+  rhocp *= M::RU;
+  // T row: d fT / d rhoY_k = (-R T sum_i hRT_i/W_i J_ik - fT cp_k) / rhocp
+  double s = 0.0;
   M::jacobian(C, T, logT, invT, g, hRT, wd, dwdT,
-              [&](int i, int k, double v) { ws[S::JS + i * N1 + k] += v; });
+              [&](int i, int k, double v) {
+                ws[S::JS + i * N1 + k] = v;
+                s += hRT[i] * M::imw(i) * v;
+              },
+              [&](int k) {
+                ws[S::JS + N * N1 + k] = (-M::RU * T * s - fT * M::RU * cpR[k] * M::imw(k)) / rhocp;
+                s = 0.0;
+              });
   double dcp[N];
   M::thermo_dcp(T, dcp);
-  double rhocp = 0.0, drhocp = 0.0, sTT = 0.0;
+  double drhocp = 0.0, sTT = 0.0;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    rhocp += y[k] * cpR[k] * M::imw(k);
     drhocp += y[k] * dcp[k] * M::imw(k);
     sTT += cpR[k] * wd[k] + hRT[k] * T * dwdT[k];
     ws[S::JS + k * N1 + N] = M::mw(k) * dwdT[k];
   }
-  rhocp *= M::RU;
   drhocp *= M::RU;
-  // T row: d fT / d rhoY_k = (-R T sum_i hRT_i/W_i J_ik - fT cp_k) / rhocp
-#pragma unroll 1
-  for (int k = 0; k < N; ++k) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < N; ++i) s += hRT[i] * M::imw(i) * ws[S::JS + i * N1 + k];
-    ws[S::JS + N * N1 + k] = (-M::RU * T * s - fT * M::RU * cpR[k] * M::imw(k)) / rhocp;
-  }
   ws[S::JS + N * N1 + N] = (-M::RU * sTT - fT * drhocp) / rhocp;
 }
 
@@ -1287,6 +1291,17 @@ __global__ void __launch_bounds__(64) k_lu_lane(const ReactArgs a, const int* __
 }
 
 
+// f(std::integral_constant<int, k>) for k = 0 .. N-1, each call with k a compile-time
+// constant (so arrays indexed by k stay in registers however long the sequence)
+template <class F, int... K>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 // Lanes per cell of the cooperative LU and solves: a 32-lane half-wave (two cells per
 // wave) up to 32 unknowns, the whole wave (one cell) up to 64.
 template <int N1>
@@ -1339,8 +1354,8 @@ __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __re
   }
   bool done = !row;
   int my_step = 0, sing = 0;
-#pragma unroll
-  for (int k = 0; k < N1; ++k) {
+  static_for<N1>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
     Real v = done ? (Real)-1 : fabs(m[k]);
     if (v != v) v = (Real)HUGE_VAL;   // a NaN row still takes a definite place in the order
     int who = r;
@@ -1354,9 +1369,9 @@ __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __re
       }
     }
     if (v <= (Real)0 && sing == 0) sing = k + 1;
-    if (row && r == who) {
+    if (row && r == who) {   // the pivot row's columns k.. (those left of k are multipliers)
 #pragma unroll
-      for (int j = 0; j < N1; ++j) prow[half][j] = m[j];
+      for (int j = k; j < N1; ++j) prow[half][j] = m[j];
     }
     __syncthreads();
     if (!done && r != who && v > (Real)0) {   // v: this column's pivot magnitude, equal in all G lanes
@@ -1373,7 +1388,7 @@ __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __re
       my_step = k;
     }
     __syncthreads();
-  }
+  });
   if (row) {
     const WS w = cell_ws<M>(a.ws, c);
     const CM mat = cell_mat<M>(a.mat, c);

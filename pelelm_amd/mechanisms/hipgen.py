@@ -234,78 +234,81 @@ def gen_wdot(mech: M.Mechanism) -> str:
 def gen_jacobian(mech: M.Mechanism) -> str:
     """Derivatives of f_i = W_i wdot_i with respect to rhoY_k (=W_k C_k) and T.
 
-    acc(i, k, v) receives v = W_i/W_k * nu_ij * dq_j/dC_k contributions; dwdT[i] the
-    molar dwdot_i/dT; wd the molar production rates at the same state.
+    Two phases, so that no Jacobian entry is accumulated in memory (a read-modify-
+    write chain whose latency would dominate): per reaction j the rate of progress q_j,
+    its T derivative and every distinct dq_j/dC_k go to write-once private arrays; then
+    wd and dwdT are formed species by species, and the entries column by column (k
+    ascending, rows i ascending, structural zeros included), each sum over reactions in
+    reaction order.  acc(i, k, v) receives each entry v = sum_j W_i/W_k nu_ij dq_j/dC_k
+    exactly once, and colend(k) follows the last row of column k.  dwdT is the molar
+    dwdot/dT and wd the molar production rates at the same state.
     """
     ns = mech.nspec
+    nr = mech.nreac
     W = mech.W
     out = []
-    out.append("template <class Acc>")
-    out.append("static __device__ __forceinline__ void jacobian(const double* __restrict__ C, const double T, const double logT, "
-               "const double invT, const double* __restrict__ gRT, const double* __restrict__ hRT, "
-               "double* __restrict__ wd, double* __restrict__ dwdT, Acc&& acc) {")
-    out.append(f"  for (int k = 0; k < {ns}; ++k) {{ wd[k] = 0.0; dwdT[k] = 0.0; }}")
-    out.append(f"  const double refC = {_d(M.P_ATM / M.R_CGS)} * invT, refCinv = 1.0 / refC;")
+    body = []
+    nu_all = []          # per reaction: {species: net stoichiometric coefficient}
+    entries: dict = {}   # (i, k) -> [(coefficient, D index)] in reaction order
+    nd = 0
     if any(r.has_third_body for r in mech.reactions):
-        out.append("  double Ctot = 0.0;")
-        out.append(f"  for (int k = 0; k < {ns}; ++k) Ctot += C[k];")
+        body.append("  double Ctot = 0.0;")
+        body.append(f"  for (int k = 0; k < {ns}; ++k) Ctot += C[k];")
     for j, r in enumerate(mech.reactions):
-        out.append(f"  {{ // {j}: {r.equation}")
-        out.append(f"    const double kf = {_kf_expr(r.A, r.beta, r.Ea / M.R_CAL)};")
-        out.append(f"    const double dlnkf = {_dlnk_expr(r.beta, r.Ea / M.R_CAL)};")
+        body.append(f"  {{ // {j}: {r.equation}")
+        body.append(f"    const double kf = {_kf_expr(r.A, r.beta, r.Ea / M.R_CAL)};")
+        body.append(f"    const double dlnkf = {_dlnk_expr(r.beta, r.Ea / M.R_CAL)};")
         if r.kind == "elementary":
-            out.append("    const double keff = kf, dkeffdT = kf * dlnkf;")
+            body.append("    const double keff = kf, dkeffdT = kf * dlnkf;")
         elif r.kind == "three-body":
-            out.append(f"    const double Mc = {_mc_expr(r, ns)};")
-            out.append("    const double keff = kf * Mc, dkeffdT = keff * dlnkf, dkeffdM = kf;")
+            body.append(f"    const double Mc = {_mc_expr(r, ns)};")
+            body.append("    const double keff = kf * Mc, dkeffdT = keff * dlnkf, dkeffdM = kf;")
         else:
             A0, b0, E0 = r.low
-            out.append(f"    const double Mc = {_mc_expr(r, ns)};")
-            out.append(f"    const double k0 = {_kf_expr(A0, b0, E0 / M.R_CAL)};")
-            out.append(f"    const double dlnk0 = {_dlnk_expr(b0, E0 / M.R_CAL)};")
-            out.append("    const double Pr = k0 * Mc / kf, dPrdM = k0 / kf, dPrdT = Pr * (dlnk0 - dlnkf);")
+            body.append(f"    const double Mc = {_mc_expr(r, ns)};")
+            body.append(f"    const double k0 = {_kf_expr(A0, b0, E0 / M.R_CAL)};")
+            body.append(f"    const double dlnk0 = {_dlnk_expr(b0, E0 / M.R_CAL)};")
+            body.append("    const double Pr = k0 * Mc / kf, dPrdM = k0 / kf, dPrdT = Pr * (dlnk0 - dlnkf);")
             if r.troe is not None:
                 fc, dfc = _troe_fc(r)
-                out.append(f"    double Fc = {fc}, dFc = {dfc};")
-                out.append("    if (Fc < 1e-300) { Fc = 1e-300; dFc = 0.0; }")
-                out.append("    const double logFc = log10(Fc);")
-                out.append("    const double cc = -0.4 - 0.67 * logFc, nn = 0.75 - 1.27 * logFc;")
-                out.append("    const double PrS = fmax(Pr, 1e-300);")
-                out.append("    const double u = log10(PrS) + cc, dd = nn - 0.14 * u, f1 = u / dd, den = 1.0 + f1 * f1;")
-                out.append("    const double F = exp(2.302585092994045684 * logFc / den);")
-                out.append("    const double df1du = nn / (dd * dd), df1dl = (-0.67 * dd + 1.1762 * u) / (dd * dd);")
-                out.append("    const double dlFdlPr = -logFc * 2.0 * f1 / (den * den) * df1du;")
-                out.append("    const double dlFdlFc = 1.0 / den - logFc * 2.0 * f1 / (den * den) * df1dl;")
-                out.append("    const double dFdPr = F * dlFdlPr / PrS, dFdT = F * dlFdlFc * dFc / Fc;")
+                body.append(f"    double Fc = {fc}, dFc = {dfc};")
+                body.append("    if (Fc < 1e-300) { Fc = 1e-300; dFc = 0.0; }")
+                body.append("    const double logFc = log10(Fc);")
+                body.append("    const double cc = -0.4 - 0.67 * logFc, nn = 0.75 - 1.27 * logFc;")
+                body.append("    const double PrS = fmax(Pr, 1e-300);")
+                body.append("    const double u = log10(PrS) + cc, dd = nn - 0.14 * u, f1 = u / dd, den = 1.0 + f1 * f1;")
+                body.append("    const double F = exp(2.302585092994045684 * logFc / den);")
+                body.append("    const double df1du = nn / (dd * dd), df1dl = (-0.67 * dd + 1.1762 * u) / (dd * dd);")
+                body.append("    const double dlFdlPr = -logFc * 2.0 * f1 / (den * den) * df1du;")
+                body.append("    const double dlFdlFc = 1.0 / den - logFc * 2.0 * f1 / (den * den) * df1dl;")
+                body.append("    const double dFdPr = F * dlFdlPr / PrS, dFdT = F * dlFdlFc * dFc / Fc;")
             else:
-                out.append("    const double F = 1.0, dFdPr = 0.0, dFdT = 0.0;")
-            out.append("    const double gg = Pr / (1.0 + Pr), dg = 1.0 / ((1.0 + Pr) * (1.0 + Pr));")
-            out.append("    const double keff = kf * gg * F;")
-            out.append("    const double dkeffdM = kf * (dg * F + gg * dFdPr) * dPrdM;")
-            out.append("    const double dkeffdT = keff * dlnkf + kf * (dg * F + gg * dFdPr) * dPrdT + kf * gg * dFdT;")
+                body.append("    const double F = 1.0, dFdPr = 0.0, dFdT = 0.0;")
+            body.append("    const double gg = Pr / (1.0 + Pr), dg = 1.0 / ((1.0 + Pr) * (1.0 + Pr));")
+            body.append("    const double keff = kf * gg * F;")
+            body.append("    const double dkeffdM = kf * (dg * F + gg * dFdPr) * dPrdM;")
+            body.append("    const double dkeffdT = keff * dlnkf + kf * (dg * F + gg * dFdPr) * dPrdT + kf * gg * dFdT;")
         pf = _prod(_power_terms(r.reactants))
-        out.append(f"    const double pf = {pf};")
+        body.append(f"    const double pf = {pf};")
         if r.reversible:
             gs, hs, dnu = _kc_exprs(r)
             pr = _prod(_power_terms(r.products))
-            out.append(f"    const double pr = {pr};")
-            out.append(f"    const double invKc = exp({gs}){_refc_factor(dnu)};")
-            out.append(f"    const double dlnKc = ({hs} - {_d(dnu)}) * invT;")
-            out.append("    const double net = pf - pr * invKc;")
-            out.append("    const double dqdT = dkeffdT * net + keff * pr * invKc * dlnKc;")
+            body.append(f"    const double pr = {pr};")
+            body.append(f"    const double invKc = exp({gs}){_refc_factor(dnu)};")
+            body.append(f"    const double dlnKc = ({hs} - {_d(dnu)}) * invT;")
+            body.append("    const double net = pf - pr * invKc;")
+            body.append("    const double dqdT = dkeffdT * net + keff * pr * invKc * dlnKc;")
         else:
-            out.append("    const double net = pf;")
-            out.append("    const double dqdT = dkeffdT * net;")
-        out.append("    const double q = keff * net;")
-        # species stoichiometry of this reaction
+            body.append("    const double net = pf;")
+            body.append("    const double dqdT = dkeffdT * net;")
+        body.append(f"    Q[{j}] = keff * net; QT[{j}] = dqdT;")
         nu: dict = {}
         for i, c in r.reactants.items():
             nu[i] = nu.get(i, 0) - c
         for i, c in r.products.items():
             nu[i] = nu.get(i, 0) + c
         nu = {i: v for i, v in nu.items() if v != 0}
-        for i, v in sorted(nu.items()):
-            out.append(f"    wd[{i}] += {_d(v)} * q; dwdT[{i}] += {_d(v)} * dqdT;")
+        nu_all.append(nu)
         # dq/dC_k from the mass-action products
         dq: dict = {}
         for k in sorted(r.reactants):
@@ -318,18 +321,49 @@ def gen_jacobian(mech: M.Mechanism) -> str:
                 e = f"{_d(coef)} * {_prod(terms)}" if coef != 1.0 else _prod(terms)
                 dq.setdefault(k, []).append(f"- keff * invKc * {e}")
         if r.has_third_body:
-            out.append("    const double tb = dkeffdM * net;")
+            body.append("    const double tb = dkeffdM * net;")
             for k in range(ns):
                 e = r.efficiencies.get(k, 1.0)
                 if e != 0.0:
                     dq.setdefault(k, []).append(f"{_d(e)} * tb" if e != 1.0 else "tb")
         for k in sorted(dq):
             expr = " + ".join(dq[k]).replace("+ -", "-")
-            out.append(f"    {{ const double d = {expr};")
+            body.append(f"    D[{nd}] = {expr};")
             for i, v in sorted(nu.items()):
-                out.append(f"      acc({i}, {k}, {_d(v * W[i] / W[k])} * d);")
-            out.append("    }")
-        out.append("  }")
+                entries.setdefault((i, k), []).append((v * W[i] / W[k], nd))
+            nd += 1
+        body.append("  }")
+    out.append("template <class Acc, class ColEnd>")
+    out.append("static __device__ __forceinline__ void jacobian(const double* __restrict__ C, const double T, const double logT, "
+               "const double invT, const double* __restrict__ gRT, const double* __restrict__ hRT, "
+               "double* __restrict__ wd, double* __restrict__ dwdT, Acc&& acc, ColEnd&& colend) {")
+    out.append(f"  const double refC = {_d(M.P_ATM / M.R_CGS)} * invT, refCinv = 1.0 / refC;")
+    out.append(f"  double Q[{nr}], QT[{nr}], D[{max(nd, 1)}];")
+    out += body
+    # production rates and their T derivatives, species by species
+    for i in range(ns):
+        terms = [(nu_all[j][i], j) for j in range(nr) if i in nu_all[j]]
+        if not terms:
+            out.append(f"  wd[{i}] = 0.0; dwdT[{i}] = 0.0;")
+            continue
+        v, j = terms[0]
+        out.append(f"  {{ double s = {_d(v)} * Q[{j}], t = {_d(v)} * QT[{j}];")
+        for v, j in terms[1:]:
+            out.append(f"    s += {_d(v)} * Q[{j}]; t += {_d(v)} * QT[{j}];")
+        out.append(f"    wd[{i}] = s; dwdT[{i}] = t; }}")
+    # Jacobian entries, column by column
+    for k in range(ns):
+        for i in range(ns):
+            terms = entries.get((i, k))
+            if not terms:
+                out.append(f"  acc({i}, {k}, 0.0);")
+                continue
+            c, m = terms[0]
+            out.append(f"  {{ double s = {_d(c)} * D[{m}];")
+            for c, m in terms[1:]:
+                out.append(f"    s += {_d(c)} * D[{m}];")
+            out.append(f"    acc({i}, {k}, s); }}")
+        out.append(f"  colend({k});")
     out.append("}")
     return "\n".join(out)
 
