@@ -366,6 +366,19 @@ struct Lane {
   __device__ void ewtset() {
     for (int i = 0; i < N1; ++i) ws[S::EWT + i] = 1.0 / (rtol * fabs(zn(0, i)) + ws[S::ATOL + i]);
   }
+  // ewtset() and then the WRMS norm of zn0 with the new weights, in one pass
+  __device__ double ewtset_wrms_zn0() {
+    double s = 0.0;
+#pragma unroll 2
+    for (int i = 0; i < N1; ++i) {
+      const double z = zn(0, i);
+      const double e = 1.0 / (rtol * fabs(z) + ws[S::ATOL + i]);
+      ws[S::EWT + i] = e;
+      const double p = z * e;
+      s += p * p;
+    }
+    return sqrt(s / N1);
+  }
   __device__ double wrms_ws(int slot) const {
     double s = 0.0;
 #pragma unroll
@@ -385,29 +398,66 @@ struct Lane {
     return sqrt(s / N1);
   }
 
-  __device__ void rescale() {
+  // The Nordsieck updates below take each component's column zn(0..q, i) into
+  // registers once, apply the same operations in the same order as the column-at-a-
+  // time loops (so the results are identical), and store it once.
+  //
+  // start of an attempt: optionally rescale (zn(j) *= eta^j, h = hscale eta) and save
+  // y_n = zn0 in TMP, then predict (tn += h; zn(j-1) += zn(j) for k = 1..q, j = q..k),
+  // and clear acor
+  __device__ void rescale_predict(const bool do_rescale, const bool save_y) {
+    double fac[LMAX];
     double factor = eta;
-    for (int j = 1; j <= q; ++j) {
+    fac[0] = 1.0;
 #pragma unroll
-      for (int i = 0; i < N1; ++i) zn(j, i) *= factor;
+    for (int j = 1; j < LMAX; ++j) {
+      fac[j] = factor;
       factor *= eta;
     }
-    h = hscale * eta;
-    hscale = h;
-  }
-  __device__ void predict() {
+    if (do_rescale) {
+      h = hscale * eta;
+      hscale = h;
+    }
     tn += h;
-    for (int k = 1; k <= q; ++k)
-      for (int j = q; j >= k; --j)
+#pragma unroll 2
+    for (int i = 0; i < N1; ++i) {
+      double z[LMAX];
 #pragma unroll
-        for (int i = 0; i < N1; ++i) zn(j - 1, i) += zn(j, i);
+      for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? zn(j, i) : 0.0;
+      if (save_y) ws[S::TMP + i] = z[0];
+      if (do_rescale) {
+#pragma unroll
+        for (int j = 1; j < LMAX; ++j)
+          if (j <= q) z[j] *= fac[j];
+      }
+#pragma unroll
+      for (int k = 1; k < LMAX; ++k)
+#pragma unroll
+        for (int j = LMAX - 1; j >= k; --j)
+          if (k <= q && j <= q) z[j - 1] += z[j];
+#pragma unroll
+      for (int j = 0; j < LMAX; ++j)
+        if (j < q || (j == q && do_rescale)) zn(j, i) = z[j];
+      ws[S::ACOR + i] = 0.0;
+    }
   }
+  // undo the prediction: zn(j-1) -= zn(j) for k = 1..q, j = q..k
   __device__ void restore() {
     tn = saved_t;
-    for (int k = 1; k <= q; ++k)
-      for (int j = q; j >= k; --j)
+#pragma unroll 2
+    for (int i = 0; i < N1; ++i) {
+      double z[LMAX];
 #pragma unroll
-        for (int i = 0; i < N1; ++i) zn(j - 1, i) -= zn(j, i);
+      for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? zn(j, i) : 0.0;
+#pragma unroll
+      for (int k = 1; k < LMAX; ++k)
+#pragma unroll
+        for (int j = LMAX - 1; j >= k; --j)
+          if (k <= q && j <= q) z[j - 1] -= z[j];
+#pragma unroll
+      for (int j = 0; j < LMAX - 1; ++j)
+        if (j < q) zn(j, i) = z[j];
+    }
   }
   __device__ void increase_bdf() {
     for (int i = 0; i <= qmax; ++i) l[i] = 0.0;
@@ -496,23 +546,17 @@ struct Lane {
   }
 
   // start one attempt of the current step: predict, coefficients, Newton set-up
-  __device__ void attempt() {
-    if (use_pred) {
-      // the state at tn (before the Nordsieck prediction) is the network's input
-#pragma unroll
-      for (int i = 0; i < N1; ++i) ws[S::TMP + i] = zn(0, i);
-      pred_state = 1;
-    } else {
-      pred_state = 0;
-    }
-    predict();
+  // (do_rescale: the step size changes by eta first)
+  __device__ void attempt(const bool do_rescale) {
+    // with the predictor, the state at tn (before the Nordsieck prediction) is the
+    // network's input
+    pred_state = use_pred ? 1 : 0;
+    rescale_predict(do_rescale, use_pred != 0);
     set_coeffs();
     convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
     callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (nst == 0) ||
                 (nst >= nstlp + MSBP) || (fabs(gamrat - 1.0) > DGMAX);
     m = 0;
-#pragma unroll
-    for (int i = 0; i < N1; ++i) ws[S::ACOR + i] = 0.0;
     a1 = 0.0; a2 = 0.0; teval = tn;
     phase = PH_NEWTON;
   }
@@ -521,23 +565,21 @@ struct Lane {
     phase = PH_NEED_CELL;
   }
   __device__ void loop_top() {
-    if (nst > 0) ewtset();
+    const double zn0_norm = (nst > 0) ? ewtset_wrms_zn0() : wrms_ws(S::ZN);
     if (nstloc >= mxstep) { finish(ST_TOO_MUCH_WORK); return; }
-    const double tolsf = UROUND * wrms_ws(S::ZN);
+    const double tolsf = UROUND * zn0_norm;
     if (tolsf > 1.0) { finish(ST_TOO_MUCH_ACC); return; }
     saved_t = tn;
     ncf = nef = 0;
     nflag = FIRST_CALL;
-    if (nst > 0 && hprime != h) {
-      if (qprime != q) {
-        adjust_order(qprime - q);
-        q = qprime;
-        L = q + 1;
-        qwait = L;
-      }
-      rescale();
+    const bool new_h = nst > 0 && hprime != h;
+    if (new_h && qprime != q) {
+      adjust_order(qprime - q);
+      q = qprime;
+      L = q + 1;
+      qwait = L;
     }
-    attempt();
+    attempt(new_h);
   }
   __device__ void begin_integration() {
     hscale = h;
@@ -605,8 +647,7 @@ struct Lane {
     if (ncf == MXNCF) { finish(ST_CONV_FAIL); return; }
     eta = ETACF;
     nflag = PREV_CONV_FAIL;
-    rescale();
-    attempt();
+    attempt(true);
   }
   __device__ void nls_fail() {
     if (pred_state == 2) {
@@ -707,8 +748,7 @@ struct Lane {
         eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / L) + ADDON);
         eta = fmax(ETAMIN, eta);
         if (nef >= SMALL_NEF) eta = fmin(eta, ETAMXF);
-        rescale();
-        attempt();
+        attempt(true);
         return false;
       }
       if (q > 1) {
@@ -717,8 +757,7 @@ struct Lane {
         L = q;
         --q;
         qwait = L;
-        rescale();
-        attempt();
+        attempt(true);
         return false;
       }
       eta = ETAMIN;
@@ -734,10 +773,12 @@ struct Lane {
     for (int i = q; i >= 2; --i) tau[i] = tau[i - 1];
     if (q == 1 && nst > 1) tau[2] = tau[1];
     tau[1] = h;
-    for (int j = 0; j <= q; ++j) {
-      const double lj = l[j];
+#pragma unroll 2
+    for (int i = 0; i < N1; ++i) {
+      const double ac = ws[S::ACOR + i];
 #pragma unroll
-      for (int i = 0; i < N1; ++i) zn(j, i) += lj * ws[S::ACOR + i];
+      for (int j = 0; j < LMAX; ++j)
+        if (j <= q) zn(j, i) += l[j] * ac;
     }
     --qwait;
     if (qwait == 1 && q != qmax) {
@@ -849,7 +890,7 @@ struct Lane {
   __device__ void restart_f(const double* f) {
 #pragma unroll
     for (int i = 0; i < N1; ++i) zn(1, i) = h * f[i];
-    attempt();
+    attempt(false);
   }
 
   __device__ void init_cell(const double* y0) {
