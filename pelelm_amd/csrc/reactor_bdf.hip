@@ -37,6 +37,12 @@
 #ifndef PELE_SETUP_WAVES
 #define PELE_SETUP_WAVES 2
 #endif
+// k_eval: two waves per SIMD (256 VGPRs) for the mechanisms up to 32 unknowns, one
+// (VGPRs plus accumulation registers for spills) above; measured on drm19 and the
+// 53-species mechanism
+#ifndef PELE_EVAL_WAVES
+#define PELE_EVAL_WAVES(N1) ((N1) <= 32 ? 2 : 1)
+#endif
 #ifndef PELE_WAVES_PER_EU
 #define PELE_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
@@ -1107,7 +1113,7 @@ __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
 
 // evaluation point y = zn0 + a1 zn1 + a2 acor at teval, and f(teval, y)
 template <class M>
-__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) k_eval(const ReactArgs a, const int* __restrict__ list,
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_EVAL_WAVES(M::N1)) k_eval(const ReactArgs a, const int* __restrict__ list,
                                                                   const unsigned int* __restrict__ count) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
