@@ -7,10 +7,20 @@ seed 104).  A step is one ReactorBase::react call over the whole box with its in
 already resident in HBM (a device-to-device restore of the inputs precedes every call
 and is inside the timed region; it moves 0.4 GB, < 1 % of a step).
 
-Forms measured (each is a complete, independently parity-tested reactor):
+Forms measured (each is a complete, independently parity-tested reactor), named
+schedule[+modifier...]:
   plain  -- FP64 BDF, fixed cell-to-lane mapping: finished cells idle until the box is done
   queue  -- FP64 BDF, compacted work lists: finished cells leave the pipeline
-`value` is the fastest form's cells/s.  The CPU baseline is this project's CPU oracle
+  auto   -- the schedule (plain/queue, with or without graph-captured trips) the reactor's
+            autotuner picks on its first call for the box size
+  +fp32  -- FP32 LU of I - gamma J with up to 3 FP64-residual refinement sweeps
+  +graphs -- trips captured into HIP graphs of 8
+  +pred  -- the learned Newton predictor, trained before the timed region on pairs the
+            plain integrator generates from cells of a seed of their own (setup time
+            reported); it runs on the queue schedule without graph capture
+The default set is the plain FP64 form and the chosen form (queue+fp32+pred).
+`value` is the fastest form's cells/s; every form reports its max |dT| against the
+plain form on all cells.  The CPU baseline is this project's CPU oracle
 (CVODE-algorithm restatement) on a bounded sample of the same cells, on the host
 threads OMP_NUM_THREADS allows.
 """
@@ -142,17 +152,25 @@ def kernel_roofline(st, nspec, mech, workload, traffic_bytes=None, form="plain")
     return line
 
 
-def run_form(form, mech, dev_in, ncell, nspec, dt, steps, warmup, profile):
+def run_form(form, mech, dev_in, ncell, nspec, dt, steps, warmup, profile, net=None):
     import torch
-    from pelelm_amd import reactor
+    from pelelm_amd import predictor, reactor
+    sched, *mods = form.split("+")
     r = reactor.ReactorBase.create("ReactorCvode")
     r.set("mi.mechanism", mech)
     r.set("ode.rtol", 1e-6)
     r.set("ode.atol", 1e-10)
     r.set("cvode.max_order", 4)
-    r.set("mi.schedule", form)
+    r.set("mi.schedule", sched)
     r.set("mi.profile", 1 if profile else 0)
+    if "fp32" in mods:
+        r.set("mi.lu", "fp32_ir")
+        r.set("mi.ir_sweeps", 3)
+    if "graphs" in mods:
+        r.set("mi.graph_trips", 8)
     r.init(2, ncell)
+    if "pred" in mods:
+        predictor.attach(r, net)
     rY = torch.empty_like(dev_in["rY"])
     T = torch.empty_like(dev_in["T"])
     rE = torch.empty_like(dev_in["rE"])
@@ -192,7 +210,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="drm19_128_mix")
     ap.add_argument("--ncell", type=int, default=None, help="override the cell count (default: the workload's)")
-    ap.add_argument("--forms", default="plain,queue")
+    ap.add_argument("--forms", default="plain,queue+fp32+pred")
+    ap.add_argument("--pred-pairs", type=int, default=131072, help="training pairs for +pred forms")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
@@ -227,11 +246,25 @@ def main(argv=None):
     log(f"generated {ncell} cells of {args.workload} in {time.time() - t0:.1f}s")
     dev_in = {k: torch.tensor(d[k], device="cuda") for k in ("rY", "T", "rE")}
 
+    net, pred_setup = None, None
+    if any("pred" in f.split("+")[1:] for f in args.forms.split(",")):
+        from pelelm_amd import predictor
+        t0 = time.time()
+        X, Tt = predictor.generate_pairs(args.workload, n=args.pred_pairs, seed=9104, groups=16, log=log)
+        t1 = time.time()
+        net = predictor.fit(d["mech"], X, Tt, epochs=8, batch=4096, log=log)
+        pred_setup = {"pairs": int(X.shape[0]), "pairs_s": t1 - t0, "train_s": time.time() - t1,
+                      "final_loss": float(net.losses[-1])}
+        log(f"predictor: {X.shape[0]} pairs in {t1 - t0:.1f}s, trained in {time.time() - t1:.1f}s")
     forms = {}
     for form in args.forms.split(","):
         forms[form] = run_form(form, d["mech"], dev_in, ncell, nspec, d["dt"], args.steps, args.warmup,
-                               profile=not args.no_profile)
+                               profile=not args.no_profile, net=net)
         log(f"{form}: {forms[form]['cells_per_s']:.4g} cells/s, {forms[form]['ms_per_step']:.1f} ms/step")
+    ref_form = "plain" if "plain" in forms else next(iter(forms))
+    T_ref = forms[ref_form]["result"]["T"]
+    for f, v in forms.items():
+        v["dT_vs_ref"] = float((v["result"]["T"] - T_ref).abs().max().item())
     best = max(forms, key=lambda f: forms[f]["cells_per_s"])
     bf = forms[best]
     st = bf["stats"]
@@ -259,12 +292,14 @@ def main(argv=None):
         else:
             n1 = n0
         log(f"cpu oracle: {n1} cells at {rate:.4g} cells/s on {nthr} threads")
-        gT = bf["result"]["T"][:n1].cpu().numpy()
-        gY = bf["result"]["rY"][:, :n1].cpu().numpy()
+        pf = forms[ref_form]   # the drop-in FP64 form is the one held to the oracle
+        gT = pf["result"]["T"][:n1].cpu().numpy()
+        gY = pf["result"]["rY"][:, :n1].cpu().numpy()
         tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
         cpu = {"value": rate, "unit": "cells/s", "cores": nthr, "kind": "port",
                "sample": f"first {n1} cells of the same {args.workload} box (same seed, dt, tolerances); "
                          f"CVODE-algorithm FP64 BDF restatement, OpenMP dynamic schedule",
+               "gpu_form_compared": ref_form,
                "gpu_vs_cpu_max_abs_dT_K": float(np.abs(gT - ref["T"]).max()),
                "gpu_vs_cpu_max_rel_err_over_tol": float((np.abs(gY - ref["rY"]) / tol).max())}
 
@@ -277,15 +312,26 @@ def main(argv=None):
         "data": f"synthetic: seeded Philox cells, fresh {wl.fuel}/air (phi U[{wl.phi[0]},{wl.phi[1]}]) blended "
                 f"with HP-equilibrium products",
         "config": {"workload": args.workload, "mechanism": d["mech"], "ncell": ncell,
-                   "box": f"{side}^3" if side ** 3 == ncell else f"{ncell} cells", "dt": d["dt"], "rtol": 1e-6, "atol": 1e-10, "max_order": 4,
-                   "form": best, "integrator": "BDF (CVODE algorithm), dense analytic-Jacobian LU, FP64"},
+                   "box": f"{side}^3" if side ** 3 == ncell else f"{ncell} cells", "dt": d["dt"], "rtol": 1e-6,
+                   "atol": 1e-10, "max_order": 4, "form": best,
+                   "integrator": "BDF (CVODE algorithm), dense analytic-Jacobian LU, FP64"
+                                 + (" rates/residuals with FP32 factors and FP64 refinement" if "fp32" in best else "")
+                                 + (", learned Newton predictor" if "pred" in best else "")},
         "forms": {f: {"cells_per_s": v["cells_per_s"], "ms_per_step": v["ms_per_step"], "nfail": v["nfail"],
+                      f"max_abs_dT_vs_{ref_form}_K": v["dT_vs_ref"],
+                      "speedup_vs_" + ref_form: v["cells_per_s"] / forms[ref_form]["cells_per_s"],
+                      "schedule_used": v["stats"].get("schedule_used"),
+                      "graph_trips_used": v["stats"].get("graph_trips_used"),
+                      "newton_iters_per_cell": v["stats"]["nni"] / max(1, v["stats"]["ncells"]),
+                      "predictions_used": v["stats"].get("npred_used"),
+                      "prediction_fallbacks": v["stats"].get("npred_fallback"),
                       "steps_per_cell": v["stats"]["nsteps"] / max(1, v["stats"]["ncells"]),
                       "rhs_per_cell": v["stats"]["nfe"] / max(1, v["stats"]["ncells"]),
                       "lu_per_cell": v["stats"]["nsetups"] / max(1, v["stats"]["ncells"]),
                       "trips": v["stats"]["ntrips"], "device_ms": v["stats"]["ms_kernel"],
                       "kernel_ms": dict(zip(KINDS, v["stats"]["kernel_ms"]))}
                   for f, v in forms.items()},
+        "predictor_setup": pred_setup,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
