@@ -130,6 +130,28 @@ int mi_chem_react_flat(void* reactor, double* rY, const double* rYsrc, double* T
 int mi_chem_reaction_rates_flat(void* reactor, const double* rY, const double* rH, double* T,
                                 const int* mask, double* rYdot, int ncells, void* hip_stream);
 
+/* PeleLM::advance_chemistry on one box -- its per-box part, Source/PeleLM.cpp:5626-5844
+   without the parallel copies: rhoY x 1e-3 and rhoH x 10 (MKS -> CGS, with the forcing
+   converted the same way, Source/PeleLM.cpp:5736-5746), react over dt from time 0
+   (Source/PeleLM.cpp:5752-5758), rhoY x 1e3 and rhoH x 0.1 back (Source/PeleLM.cpp:
+   5764-5771), and I_R = -(rhoY_old - rhoY_new) / dt - F_Y (Source/PeleLM.cpp:5798-5802).
+   s_old / s_new: rhoY (nspec components), rhoH, T at components 0 .. nspec+1 of each view
+   (the state from first_spec on); s_old is read only and must not share memory with
+   s_new, whose rhoY, rhoH and T are written.  force: F_Y (nspec), F_h (MKS, read only);
+   rhoYdot: nspec components written; FC: one component written (RHS evaluations).
+   Asynchronous on hip_stream; the forcing's CGS copy lives in a reactor-owned buffer
+   (grown on a larger box, kept between calls). */
+int mi_chem_advance_box(void* reactor, const int lo[3], const int hi[3], mi_real4 s_old, mi_real4 s_new,
+                        mi_real4 force, mi_real4 rhoYdot, mi_real4 FC, mi_int4 mask, double dt,
+                        void* hip_stream);
+
+/* PeleLM::RhoH_to_Temp on one box (getTfromHY, Source/PeleLM_K.H:511-530, called at
+   Source/PeleLM.cpp:5176): Y = rhoY / rho, h = rhoH / rho x 1e4 (MKS to erg/g), and T
+   re-solved in place by HY2T starting from the T it holds.  rho: 1 component, rhoY:
+   nspec, rhoH: 1, T: 1 (read and written). */
+int mi_chem_rhoh_to_temp_box(void* reactor, const int lo[3], const int hi[3], mi_real4 rho, mi_real4 rhoY,
+                             mi_real4 rhoH, mi_real4 T, void* hip_stream);
+
 /* Statistics of the last react call (synchronises the stream of that call). */
 int mi_chem_get_stats(void* reactor, mi_chem_stats* out);
 

@@ -60,6 +60,8 @@ struct Reactor {
   double* mat = nullptr;        // per-cell matrix region (factors, Newton vectors)
   int* lists = nullptr;         // 6 lists of ws_cells ints (active x2, set-up x2, predictor, solve)
   long ws_cells = 0;            // cells the workspace and lists hold
+  double* glue = nullptr;       // advance_box: the forcing in CGS, (nspec + 1) x glue_cells
+  long glue_cells = 0;
   unsigned int* counts = nullptr;
   unsigned long long* counters = nullptr;
   long last_trips = 0;
@@ -86,6 +88,9 @@ void free_workspace(Reactor* r) {
   if (r->iws) (void)hipFree(r->iws);
   if (r->lists) (void)hipFree(r->lists);
   if (r->mat) (void)hipFree(r->mat);
+  if (r->glue) (void)hipFree(r->glue);
+  r->glue = nullptr;
+  r->glue_cells = 0;
   r->mat = nullptr;
   r->ws = nullptr;
   r->iws = nullptr;
@@ -467,6 +472,88 @@ int mi_chem_reaction_rates_flat(void* reactor, const double* rY, const double* r
   if (ncells < 0 || !rY || !rH || !T || !rYdot) return fail(r, MI_EINVAL, "bad arguments");
   if (ncells == 0) return MI_OK;
   HIPCHK(r, launch_rates(r->mech, rY, rH, T, mask, rYdot, ncells, static_cast<hipStream_t>(hip_stream)));
+  return MI_OK;
+}
+
+int mi_chem_advance_box(void* reactor, const int lo[3], const int hi[3], mi_real4 s_old, mi_real4 s_new,
+                        mi_real4 force, mi_real4 rhoYdot, mi_real4 FC, mi_int4 mask, double dt, void* hip_stream) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (!r->inited) return fail(r, MI_ESTATE, "advance_box called before init");
+  if (!lo || !hi) return fail(r, MI_EINVAL, "lo and hi are required");
+  if (!(dt > 0.0)) return fail(r, MI_EINVAL, "dt must be positive");
+  for (int d = 0; d < 3; ++d)
+    if (hi[d] < lo[d]) return fail(r, MI_EINVAL, "empty box");
+  const int N = mech_nspec(r->mech);
+  if (!s_old.p || !s_new.p || !force.p || !rhoYdot.p || !FC.p)
+    return fail(r, MI_EINVAL, "s_old, s_new, force, rhoYdot and FC are required");
+  if (!box_covers(s_old, lo, hi, N + 2) || !box_covers(s_new, lo, hi, N + 2) || !box_covers(force, lo, hi, N + 1) ||
+      !box_covers(rhoYdot, lo, hi, N) || !box_covers(FC, lo, hi, 1))
+    return fail(r, MI_EINVAL, "an array view does not cover the box or has too few components");
+  {  // s_old is read after the reaction (I_R): it must not overlap s_new's storage
+    const double* a0 = s_old.p;
+    const double* a1 = s_old.p + (long)(s_old.ncomp) * s_old.nstride;
+    const double* b0 = s_new.p;
+    const double* b1 = s_new.p + (long)(s_new.ncomp) * s_new.nstride;
+    if (a0 < b1 && b0 < a1) return fail(r, MI_EINVAL, "s_old and s_new must not share memory");
+  }
+  const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;
+  const long ncell = (long)nx * ny * (hi[2] - lo[2] + 1);
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  if (ncell > r->glue_cells) {
+    if (r->glue) HIPCHK(r, hipFree(r->glue));
+    r->glue = nullptr;
+    r->glue_cells = 0;
+    if (hipMalloc(&r->glue, sizeof(double) * (N + 1) * (size_t)ncell) != hipSuccess)
+      return fail(r, MI_ENOMEM, "glue buffer allocation failed");
+    r->glue_cells = ncell;
+  }
+  const ArrView vo = box_view(s_old, lo), vn = box_view(s_new, lo), vf = box_view(force, lo);
+  HIPCHK(r, launch_glue_in(vo, vn, vf, r->glue, N, nx, ny, ncell, st));
+  // the forcing's CGS copy as a flat box-shaped view (cell fastest)
+  mi_real4 fy{};
+  fy.p = r->glue;
+  for (int d = 0; d < 3; ++d) {
+    fy.begin[d] = lo[d];
+    fy.end[d] = hi[d] + 1;
+  }
+  fy.jstride = nx;
+  fy.kstride = (long)nx * ny;
+  fy.nstride = ncell;
+  fy.ncomp = N + 1;
+  mi_real4 fh = fy;
+  fh.p = r->glue + (long)N * ncell;
+  fh.ncomp = 1;
+  auto comp = [](mi_real4 v, int n) {
+    v.p += (long)n * v.nstride;
+    v.ncomp -= n;
+    return v;
+  };
+  double dt_io = dt, time = 0.0;
+  const int rc = mi_chem_react_box(reactor, lo, hi, s_new, fy, comp(s_new, N + 1), comp(s_new, N), fh, FC, mask,
+                                   &dt_io, &time, hip_stream);
+  if (rc != MI_OK && rc != MI_ECELLFAIL) return rc;
+  const ArrView vd = box_view(rhoYdot, lo);
+  HIPCHK(r, launch_glue_out(vo, vn, vf, vd, 1.0 / dt, N, nx, ny, ncell, st));
+  return rc;
+}
+
+int mi_chem_rhoh_to_temp_box(void* reactor, const int lo[3], const int hi[3], mi_real4 rho, mi_real4 rhoY,
+                             mi_real4 rhoH, mi_real4 T, void* hip_stream) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (!lo || !hi) return fail(r, MI_EINVAL, "lo and hi are required");
+  for (int d = 0; d < 3; ++d)
+    if (hi[d] < lo[d]) return fail(r, MI_EINVAL, "empty box");
+  const int N = mech_nspec(r->mech);
+  if (!rho.p || !rhoY.p || !rhoH.p || !T.p) return fail(r, MI_EINVAL, "rho, rhoY, rhoH and T are required");
+  if (!box_covers(rho, lo, hi, 1) || !box_covers(rhoY, lo, hi, N) || !box_covers(rhoH, lo, hi, 1) ||
+      !box_covers(T, lo, hi, 1))
+    return fail(r, MI_EINVAL, "an array view does not cover the box or has too few components");
+  const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;
+  const long ncell = (long)nx * ny * (hi[2] - lo[2] + 1);
+  HIPCHK(r, launch_rhoh_to_temp(r->mech, box_view(rho, lo), box_view(rhoY, lo), box_view(rhoH, lo),
+                                box_view(T, lo), nx, ny, ncell, static_cast<hipStream_t>(hip_stream)));
   return MI_OK;
 }
 

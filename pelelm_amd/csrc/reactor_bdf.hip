@@ -2154,6 +2154,33 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
   return hipSuccess;
 }
 
+// PeleLM::RhoH_to_Temp per cell (getTfromHY, Source/PeleLM_K.H:511-530): Y = rhoY / rho
+// and h = rhoH / rho x 1e4 (MKS to erg/g), T re-solved by HY2T from the incoming T
+template <class M>
+__global__ void __launch_bounds__(64) rhoh_to_temp_kernel(const ArrView rho, const ArrView rhoY, const ArrView rhoH,
+                                                          const ArrView T, const int nx, const int ny, const long ncell) {
+  constexpr int N = M::NS;
+  const long c = (long)blockIdx.x * 64 + threadIdx.x;
+  if (c >= ncell) return;
+  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long)nx * ny));
+  const double rhoinv = 1.0 / rho.p[cell_off(rho, i, j, k)];
+  double y[N];
+  const long oy = cell_off(rhoY, i, j, k);
+#pragma unroll
+  for (int n = 0; n < N; ++n) y[n] = rhoY.p[oy + n * rhoY.ns] * rhoinv;
+  const double h_cgs = rhoH.p[cell_off(rhoH, i, j, k)] * rhoinv * 1.0e4;
+  const long ot = cell_off(T, i, j, k);
+  T.p[ot] = hy2t<M>(h_cgs, y, T.p[ot]);
+}
+
+template <class M>
+hipError_t launch_rhoh_to_temp_t(const ArrView& rho, const ArrView& rhoY, const ArrView& rhoH, const ArrView& T,
+                                 int nx, int ny, long ncell, hipStream_t stream) {
+  hipLaunchKernelGGL(rhoh_to_temp_kernel<M>, dim3((unsigned)((ncell + 63) / 64)), dim3(64), 0, stream, rho, rhoY,
+                     rhoH, T, nx, ny, ncell);
+  return hipGetLastError();
+}
+
 template <class M>
 hipError_t launch_rates_t(const double* rY, const double* rH, double* T, const int* mask, double* rYdot, long ncell,
                           hipStream_t stream) {
@@ -2233,6 +2260,68 @@ hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T,
     case MECH_DODECANE53: return launch_rates_t<mech_dodecane53>(rY, rH, T, mask, rYdot, ncell, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_rhoh_to_temp(int mech, const ArrView& rho, const ArrView& rhoY, const ArrView& rhoH,
+                               const ArrView& T, int nx, int ny, long ncell, hipStream_t stream) {
+  switch (mech) {
+    case MECH_LIDRYER: return launch_rhoh_to_temp_t<mech_LiDryer>(rho, rhoY, rhoH, T, nx, ny, ncell, stream);
+    case MECH_DRM19: return launch_rhoh_to_temp_t<mech_drm19>(rho, rhoY, rhoH, T, nx, ny, ncell, stream);
+    case MECH_DODECANE53: return launch_rhoh_to_temp_t<mech_dodecane53>(rho, rhoY, rhoH, T, nx, ny, ncell, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// PeleLM::advance_chemistry's per-box glue (Source/PeleLM.cpp:5736-5746, 5764-5771,
+// 5798-5802).  In: the new state's rhoY (x 1e-3) and rhoH (x 10) from the old state, T
+// copied; the forcing converted the same way into a flat cell-fastest buffer.
+__global__ void glue_in_kernel(const ArrView sold, const ArrView snew, const ArrView force, double* __restrict__ fcgs,
+                               const int nspec, const int nx, const int ny, const long ncell) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long)nx * ny));
+  const long oo = cell_off(sold, i, j, k), on = cell_off(snew, i, j, k), of = cell_off(force, i, j, k);
+  for (int n = 0; n < nspec; ++n) {
+    snew.p[on + n * snew.ns] = sold.p[oo + n * sold.ns] * 1.0e-3;
+    fcgs[n * ncell + c] = force.p[of + n * force.ns] * 1.0e-3;
+  }
+  snew.p[on + nspec * snew.ns] = sold.p[oo + nspec * sold.ns] * 10.0;
+  fcgs[nspec * ncell + c] = force.p[of + nspec * force.ns] * 10.0;
+  snew.p[on + (nspec + 1) * snew.ns] = sold.p[oo + (nspec + 1) * sold.ns];
+}
+
+// Out: rhoY (x 1e3) and rhoH (x 0.1) back to MKS, and I_R = -(rhoY_old - rhoY_new) / dt - F_Y
+__global__ void glue_out_kernel(const ArrView sold, const ArrView snew, const ArrView force, const ArrView rdot,
+                                const double dt_inv, const int nspec, const int nx, const int ny, const long ncell) {
+  // every product rounded on its own: I_R is formed from the stored new state (the
+  // reference converts, stores, then forms I_R in a separate pass), so the x 1e3 must
+  // not fuse into the difference
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long)nx * ny));
+  const long oo = cell_off(sold, i, j, k), on = cell_off(snew, i, j, k), of = cell_off(force, i, j, k);
+  const long od = cell_off(rdot, i, j, k);
+  for (int n = 0; n < nspec; ++n) {
+    const double yn = snew.p[on + n * snew.ns] * 1.0e3;
+    snew.p[on + n * snew.ns] = yn;
+    rdot.p[od + n * rdot.ns] = -(sold.p[oo + n * sold.ns] - yn) * dt_inv - force.p[of + n * force.ns];
+  }
+  snew.p[on + nspec * snew.ns] *= 0.1;
+}
+
+hipError_t launch_glue_in(const ArrView& sold, const ArrView& snew, const ArrView& force, double* fcgs, int nspec,
+                          int nx, int ny, long ncell, hipStream_t stream) {
+  hipLaunchKernelGGL(glue_in_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, stream, sold, snew, force,
+                     fcgs, nspec, nx, ny, ncell);
+  return hipGetLastError();
+}
+
+hipError_t launch_glue_out(const ArrView& sold, const ArrView& snew, const ArrView& force, const ArrView& rdot,
+                           double dt_inv, int nspec, int nx, int ny, long ncell, hipStream_t stream) {
+  hipLaunchKernelGGL(glue_out_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, stream, sold, snew, force,
+                     rdot, dt_inv, nspec, nx, ny, ncell);
+  return hipGetLastError();
 }
 
 hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t stream) {

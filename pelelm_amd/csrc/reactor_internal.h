@@ -111,6 +111,12 @@ hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T,
                         long ncell, hipStream_t stream);
 hipError_t launch_null(int nspec, const ReactArgs& a, hipStream_t stream);
 // copy a call's rY [nspec][ncell], T and rE [ncell] into flat device arrays
+hipError_t launch_rhoh_to_temp(int mech, const ArrView& rho, const ArrView& rhoY, const ArrView& rhoH,
+                               const ArrView& T, int nx, int ny, long ncell, hipStream_t stream);
+hipError_t launch_glue_in(const ArrView& sold, const ArrView& snew, const ArrView& force, double* fcgs, int nspec,
+                          int nx, int ny, long ncell, hipStream_t stream);
+hipError_t launch_glue_out(const ArrView& sold, const ArrView& snew, const ArrView& force, const ArrView& rdot,
+                           double dt_inv, int nspec, int nx, int ny, long ncell, hipStream_t stream);
 hipError_t launch_gather_state(int nspec, const ReactArgs& a, double* rY, double* T, double* rE, hipStream_t stream);
 
 }  // namespace pelelm

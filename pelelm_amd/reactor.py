@@ -46,8 +46,8 @@ class MiStats(C.Structure):
 
 # Every exported entry point of include/pelelm_chem.h.
 EXPORTS = ["mi_chem_create", "mi_chem_init", "mi_chem_set", "mi_chem_set_typ_vals", "mi_chem_react_box",
-           "mi_chem_react_flat", "mi_chem_reaction_rates_flat", "mi_chem_get_stats", "mi_chem_nspec",
-           "mi_chem_last_error", "mi_chem_close"]
+           "mi_chem_react_flat", "mi_chem_reaction_rates_flat", "mi_chem_advance_box", "mi_chem_rhoh_to_temp_box",
+           "mi_chem_get_stats", "mi_chem_nspec", "mi_chem_last_error", "mi_chem_close"]
 
 _lib = None
 
@@ -69,6 +69,9 @@ def lib():
         L.mi_chem_react_flat.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), vp]
         L.mi_chem_reaction_rates_flat.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, vp]
+        L.mi_chem_advance_box.argtypes = [vp, C.c_int * 3, C.c_int * 3, MiReal4, MiReal4, MiReal4, MiReal4, MiReal4,
+                                          MiInt4, C.c_double, vp]
+        L.mi_chem_rhoh_to_temp_box.argtypes = [vp, C.c_int * 3, C.c_int * 3, MiReal4, MiReal4, MiReal4, MiReal4, vp]
         L.mi_chem_get_stats.argtypes = [vp, C.POINTER(MiStats)]
         L.mi_chem_nspec.argtypes = [vp]
         L.mi_chem_last_error.argtypes = [vp]
@@ -172,6 +175,19 @@ class ReactorBase:
         self._check(lib().mi_chem_react_box(self._h, (C.c_int * 3)(*lo), (C.c_int * 3)(*hi), rY, rYsrc, T, rE,
                                             rEsrc, FC, mask, C.byref(d), C.byref(tm), _stream(stream)))
         return d.value, tm.value
+
+    def advance_box(self, lo, hi, s_old: MiReal4, s_new: MiReal4, force: MiReal4, rhoYdot: MiReal4, FC: MiReal4,
+                    mask: MiInt4, dt: float, stream=None):
+        """PeleLM::advance_chemistry on one box (Source/PeleLM.cpp:5626-5844, per-box part):
+        MKS state in s_old (rhoY, rhoH, T from component 0), new state into s_new, I_R into
+        rhoYdot, RHS counts into FC."""
+        self._check(lib().mi_chem_advance_box(self._h, (C.c_int * 3)(*lo), (C.c_int * 3)(*hi), s_old, s_new, force,
+                                              rhoYdot, FC, mask, float(dt), _stream(stream)))
+
+    def rhoh_to_temp_box(self, lo, hi, rho: MiReal4, rhoY: MiReal4, rhoH: MiReal4, T: MiReal4, stream=None):
+        """PeleLM::RhoH_to_Temp on one box (getTfromHY, Source/PeleLM_K.H:511-530), MKS."""
+        self._check(lib().mi_chem_rhoh_to_temp_box(self._h, (C.c_int * 3)(*lo), (C.c_int * 3)(*hi), rho, rhoY, rhoH,
+                                                   T, _stream(stream)))
 
     def reaction_rates(self, rY, rH, T, mask, rYdot, stream=None):
         ncell = T.shape[0]

@@ -94,3 +94,42 @@ def test_react_before_init_is_a_state_error():
     rc = lib.mi_chem_react_flat(h, None, None, None, None, None, None, None, 0, C.byref(dt), C.byref(tm), None)
     assert rc == reactor.MI_EINVAL
     assert lib.mi_chem_close(h) == 0
+
+
+def test_advance_box_argument_errors():
+    """The glue entry point checks its views before touching the device: call order,
+    coverage of the box, and old and new state sharing memory."""
+    lib = reactor.lib()
+    h = C.c_void_p()
+    assert lib.mi_chem_create(b"ReactorCvode", C.byref(h)) == 0
+    buf = (C.c_double * (23 * 8))()
+    ibuf = (C.c_int * 8)()
+
+    def view(off, ncomp):
+        v = reactor.MiReal4()
+        v.p = C.cast(C.byref(buf, 8 * off), C.c_void_p).value
+        v.begin = (C.c_int * 3)(0, 0, 0)
+        v.end = (C.c_int * 3)(2, 2, 2)
+        v.jstride, v.kstride, v.nstride, v.ncomp = 2, 4, 8, ncomp
+        return v
+    m = reactor.MiInt4()
+    m.p = C.cast(ibuf, C.c_void_p).value
+    lo, hi = (C.c_int * 3)(0, 0, 0), (C.c_int * 3)(1, 1, 1)
+    rc = lib.mi_chem_advance_box(h, lo, hi, view(0, 23), view(0, 23), view(0, 22), view(0, 21), view(0, 1), m, 1e-6,
+                                 None)
+    assert rc == reactor.MI_ESTATE
+    assert lib.mi_chem_init(h, 2, 1) == 0
+    # drm19 (21 species): a new-state view with too few components
+    rc = lib.mi_chem_advance_box(h, lo, hi, view(0, 23), view(8, 21), view(0, 22), view(0, 21), view(0, 1), m, 1e-6,
+                                 None)
+    assert rc == reactor.MI_EINVAL and b"cover" in lib.mi_chem_last_error(h)
+    # old and new state in the same memory
+    rc = lib.mi_chem_advance_box(h, lo, hi, view(0, 23), view(0, 23), view(0, 22), view(0, 21), view(0, 1), m, 1e-6,
+                                 None)
+    assert rc == reactor.MI_EINVAL and b"share memory" in lib.mi_chem_last_error(h)
+    big = (C.c_double * (23 * 8))()
+    other = view(0, 23)
+    other.p = C.cast(big, C.c_void_p).value
+    rc = lib.mi_chem_advance_box(h, lo, hi, view(0, 23), other, view(0, 22), view(0, 21), view(0, 1), m, -1.0, None)
+    assert rc == reactor.MI_EINVAL and b"dt" in lib.mi_chem_last_error(h)
+    assert lib.mi_chem_close(h) == 0
