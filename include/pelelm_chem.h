@@ -91,9 +91,9 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
    (Exec/RegTests/FlameSheet/inputs.3d-regt:84-90) plus this reactor's own:
      ode.rtol, ode.atol, cvode.max_order, cvode.max_num_steps (default 20000),
      cvode.solve_type (denseAJ_direct), peleLM.use_typ_vals_chem (0/1),
-     mi.mechanism (drm19 | LiDryer | dodecane53), mi.schedule (queue: finished cells leave the work
-     lists, default | plain: fixed cell-to-lane mapping for the evaluation and step
-     kernels | auto: the first call of each box size times both, with and without graph
+     mi.mechanism (drm19 | LiDryer | dodecane53), mi.schedule (plain: fixed cell-to-lane mapping
+     for the evaluation and step kernels, default | queue: finished cells leave the work
+     lists | auto: the first call of each box size times both, with and without graph
      capture, on a copy of its cells and keeps the fastest), mi.graph_trips (0: the host
      issues every trip, default | K even: K trips per captured HIP graph), mi.lu (fp64 |
      fp32_ir: FP32 factors with FP64-residual refinement), mi.ir_sweeps (3), mi.lu_form

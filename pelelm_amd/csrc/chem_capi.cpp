@@ -35,7 +35,7 @@ struct Reactor {
   bool use_typ = false;
   std::vector<double> typ;
   // this reactor's keys
-  int schedule = SCHED_COMPACT;
+  int schedule = SCHED_PLAIN;     // measured fastest on every workload here
   int lu_mode = LU_FP64;
   int ir_sweeps = 3;
   int lu_lane = 0;

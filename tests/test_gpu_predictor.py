@@ -24,6 +24,7 @@ def _run(d, net):
     n = d["T"].shape[0]
     r = reactor.ReactorBase.create("ReactorCvode")
     r.set("mi.mechanism", "drm19")
+    r.set("mi.schedule", "queue")   # the learned predictor's work list
     r.set("ode.rtol", 1e-6)
     r.set("ode.atol", 1e-10)
     r.init(2, 1)
