@@ -1403,25 +1403,34 @@ __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __re
   int my_step = 0, sing = 0;
   static_for<N1>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    Real v = done ? (Real)-1 : fabs(m[k]);
-    if (v != v) v = (Real)HUGE_VAL;   // a NaN row still takes a definite place in the order
-    int who = r;
+    // Pivot: the largest |m_rk| over the remaining rows, ties to the lowest row.  The
+    // bits of a non-negative double order as unsigned integers, so the search is one
+    // 64-bit max over the cell's lanes and a ballot for the lowest lane holding it;
+    // remaining rows carry bits + 1 so that a zero entry still beats a finished row, and
+    // a NaN row counts as infinite (it still takes a definite place in the order).
+    unsigned long long key = 0ull;
+    if (!done) {
+      double av = fabs((double)m[k]);
+      if (av != av) av = HUGE_VAL;
+      key = (unsigned long long)__double_as_longlong(av) + 1ull;
+    }
+    unsigned long long kmax = key;
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) {
-      const Real ov = __shfl_xor(v, off, G);
-      const int ow = __shfl_xor(who, off, G);
-      if (ov > v || (ov == v && ow < who)) {
-        v = ov;
-        who = ow;
-      }
+      const unsigned long long o = __shfl_xor(kmax, off, G);
+      kmax = o > kmax ? o : kmax;
     }
-    if (v <= (Real)0 && sing == 0) sing = k + 1;
+    const unsigned long long ball = __ballot(key == kmax);
+    const unsigned long long mine = (G == 64) ? ball : ((ball >> (half * 32)) & 0xFFFFFFFFull);
+    const int who = __ffsll((long long)mine) - 1;
+    const bool nonzero = kmax > 1ull;   // the pivot magnitude is > 0 (equal in all G lanes)
+    if (!nonzero && sing == 0) sing = k + 1;
     if (row && r == who) {   // the pivot row's columns k.. (those left of k are multipliers)
 #pragma unroll
       for (int j = k; j < N1; ++j) prow[half][j] = m[j];
     }
     __syncthreads();
-    if (!done && r != who && v > (Real)0) {   // v: this column's pivot magnitude, equal in all G lanes
+    if (!done && r != who && nonzero) {
       const Real inv = (Real)1 / prow[half][k];
       const Real l = m[k] * inv;
       m[k] = l;

@@ -65,6 +65,7 @@ inline void __syncthreads() {}
 #define __shared__ static
 inline unsigned long long __ballot(int p) { return p ? (1ull << __lane_id()) : 0ull; }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
+inline long long __double_as_longlong(double x) { long long r; __builtin_memcpy(&r, &x, 8); return r; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 template <class T> inline T atomicAdd(T* p, T v) { T o = *p; *p += v; return o; }
 template <class T> inline T atomicMax(T* p, T v) { T o = *p; if (v > o) *p = v; return o; }
