@@ -1349,6 +1349,32 @@ __device__ __forceinline__ void static_for(F&& f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
+// x as held by lane `src` (a compile-time constant) of this lane's group of G lanes.
+// A whole-wave group reads it with v_readlane (a scalar broadcast, measured faster
+// than a permute through the LDS crossbar); two groups per wave would need both
+// lanes read and a select, which measured no faster than the permute it replaces.
+template <int G>
+__device__ __forceinline__ double group_bcast(const double x, const int src) {
+  if (G == 64) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  }
+  return __shfl(x, src, G);
+}
+
+// x as held by lane `src` of the wave (src uniform, possibly known only at run time)
+__device__ __forceinline__ double lane_read(const double x, const int src) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_read(const float x, const int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+
 // Lanes per cell of the cooperative LU and solves: a 32-lane half-wave (two cells per
 // wave) up to 32 unknowns, the whole wave (one cell) up to 64.
 template <int N1>
@@ -1425,25 +1451,43 @@ __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __re
     const int who = __ffsll((long long)mine) - 1;
     const bool nonzero = kmax > 1ull;   // the pivot magnitude is > 0 (equal in all G lanes)
     if (!nonzero && sing == 0) sing = k + 1;
-    if (row && r == who) {   // the pivot row's columns k.. (those left of k are multipliers)
-#pragma unroll
-      for (int j = k; j < N1; ++j) prow[half][j] = m[j];
-    }
-    __syncthreads();
-    if (!done && r != who && nonzero) {
-      const Real inv = (Real)1 / prow[half][k];
-      const Real l = m[k] * inv;
-      m[k] = l;
-      if (l != (Real)0) {
-#pragma unroll
-        for (int j = k + 1; j < N1; ++j) m[j] -= l * prow[half][j];
+    if constexpr (G == 64 && !IS32) {
+      // one cell per wave, FP64: the pivot row's entries are read from lane `who`
+      // (uniform) with v_readlane; that lane's row is not updated in this step.  (The
+      // FP32 factorisation measured faster with the LDS broadcast below.)
+      const bool upd = !done && r != who && nonzero;
+      const Real pk = lane_read(m[k], who);
+      Real l = (Real)0;
+      if (upd) {
+        l = m[k] * ((Real)1 / pk);
+        m[k] = l;
       }
+#pragma unroll
+      for (int j = k + 1; j < N1; ++j) {
+        const Real pj = lane_read(m[j], who);
+        if (upd && l != (Real)0) m[j] -= l * pj;
+      }
+    } else {
+      if (row && r == who) {   // the pivot row's columns k.. (those left of k are multipliers)
+#pragma unroll
+        for (int j = k; j < N1; ++j) prow[half][j] = m[j];
+      }
+      __syncthreads();
+      if (!done && r != who && nonzero) {
+        const Real inv = (Real)1 / prow[half][k];
+        const Real l = m[k] * inv;
+        m[k] = l;
+        if (l != (Real)0) {
+#pragma unroll
+          for (int j = k + 1; j < N1; ++j) m[j] -= l * prow[half][j];
+        }
+      }
+      __syncthreads();
     }
     if (r == who && !done) {
       done = true;
       my_step = k;
     }
-    __syncthreads();
   });
   if (row) {
     const WS w = cell_ws<M>(a.ws, c);
@@ -1575,13 +1619,13 @@ __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* _
   auto subst = [&](double xv) {
 #pragma unroll
     for (int k = 0; k < N1 - 1; ++k) {
-      const double xk = __shfl(xv, k, G);
+      const double xk = group_bcast<G>(xv, k);
       if (r > k) xv -= (double)lrow[k] * xk;
     }
 #pragma unroll
     for (int k = N1 - 1; k >= 0; --k) {
       if (r == k) xv /= (double)lrow[k];
-      const double xk = __shfl(xv, k, G);
+      const double xk = group_bcast<G>(xv, k);
       if (r < k) xv -= (double)lrow[k] * xk;
     }
     return row ? xv : 0.0;
@@ -1601,7 +1645,7 @@ __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* _
       double res = bi - x;
 #pragma unroll
       for (int j = 0; j < N1; ++j) {
-        const double xj = __shfl(x, j, G);
+        const double xj = group_bcast<G>(x, j);
         if (row) res += gp * mat[MS::JS + r * N1 + j] * xj;
       }
       const double d = subst(__shfl(res, pr, G));
