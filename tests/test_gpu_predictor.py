@@ -83,3 +83,42 @@ def test_trained_predictor_keeps_parity_and_reports_its_gain():
     assert res["trained"]["npred_fallback"] <= res["untrained"]["npred_fallback"]
     net.close()
     untrained.close()
+
+
+def test_predictor_on_the_53_species_mechanism():
+    """The same network shape for the 53-species mechanism (56 inputs padded to 64, 54
+    outputs padded to 64 on the matrix cores): trained on its own pairs, attached, it
+    runs the stiffness-mix cells to completion with every cell integrated, uses its
+    predictions, and stays within the integration error's order of magnitude."""
+    X, Tt = predictor.generate_pairs("dodecane53_64_mix", n=32768, seed=9105, groups=8)
+    net = predictor.fit("dodecane53", X, Tt, epochs=6, batch=4096)
+    assert X.shape[1] == 56 and Tt.shape[1] == 54
+    assert net.losses[-1] < net.losses[0]
+    d = cells.generate_workload("dodecane53_64_mix", ncell=2048)
+    m = O.Mech("dodecane53")
+    tight = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-12, atol=1e-20, max_steps=200000)
+    out = {}
+    for name, nn in (("plain", None), ("trained", net)):
+        n = d["T"].shape[0]
+        r = reactor.ReactorBase.create("ReactorCvode")
+        r.set("mi.mechanism", "dodecane53")
+        r.set("mi.schedule", "queue")
+        r.set("ode.rtol", 1e-6)
+        r.set("ode.atol", 1e-10)
+        r.init(2, 1)
+        if nn is not None:
+            predictor.attach(r, nn)
+        rY, T, rE = (torch.tensor(d[k], device="cuda") for k in ("rY", "T", "rE"))
+        FC = torch.zeros(n, dtype=torch.float64, device="cuda")
+        r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+        torch.cuda.synchronize()
+        st = r.stats()
+        r.close()
+        out[name] = (np.abs(T.cpu().numpy() - tight["T"]).max(), st)
+        print(f"53 species {name}: max|dT| vs tight oracle {out[name][0]:.3g} K, Newton iterations {st['nni']}, "
+              f"predictions {st['npred_used']}, fallbacks {st['npred_fallback']}")
+    for name, (_, st) in out.items():
+        assert st["nfail"] == 0 and st["ncells"] == 2048, name
+    assert out["trained"][1]["npred_used"] > 0
+    assert out["trained"][0] <= 20.0 * out["plain"][0] + 1.0
+    net.close()
