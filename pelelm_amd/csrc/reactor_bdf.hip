@@ -1349,30 +1349,23 @@ __device__ __forceinline__ void static_for(F&& f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-// x as held by lane `src` (a compile-time constant) of this lane's group of G lanes.
-// A whole-wave group reads it with v_readlane (a scalar broadcast, measured faster
-// than a permute through the LDS crossbar); two groups per wave would need both
-// lanes read and a select, which measured no faster than the permute it replaces.
-template <int G>
-__device__ __forceinline__ double group_bcast(const double x, const int src) {
-  if (G == 64) {
-    const long long b = __double_as_longlong(x);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, src);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), src);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-  }
-  return __shfl(x, src, G);
-}
-
-// x as held by lane `src` of the wave (src uniform, possibly known only at run time)
+// x as held by lane `src` of the wave (src uniform, possibly known only at run time),
+// read with v_readlane: a scalar broadcast, no trip through the LDS crossbar
 __device__ __forceinline__ double lane_read(const double x, const int src) {
   const long long b = __double_as_longlong(x);
   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, src);
   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), src);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-__device__ __forceinline__ float lane_read(const float x, const int src) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+
+// x as held by lane `src` (a compile-time constant) of this lane's group of G lanes:
+// v_readlane for a whole-wave group (measured faster than a permute); two groups per
+// wave would need both lanes read and a select, which measured no faster than the
+// permute kept for them
+template <int G>
+__device__ __forceinline__ double group_bcast(const double x, const int src) {
+  if (G == 64) return lane_read(x, src);
+  return __shfl(x, src, G);
 }
 
 // Lanes per cell of the cooperative LU and solves: a 32-lane half-wave (two cells per

@@ -68,8 +68,6 @@ inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline long long __double_as_longlong(double x) { long long r; __builtin_memcpy(&r, &x, 8); return r; }
 inline double __longlong_as_double(long long x) { double r; __builtin_memcpy(&r, &x, 8); return r; }
 #define __builtin_amdgcn_readlane(v, l) (v)
-inline float __int_as_float(int x) { float r; __builtin_memcpy(&r, &x, 4); return r; }
-inline int __float_as_int(float x) { int r; __builtin_memcpy(&r, &x, 4); return r; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 template <class T> inline T atomicAdd(T* p, T v) { T o = *p; *p += v; return o; }
 template <class T> inline T atomicMax(T* p, T v) { T o = *p; if (v > o) *p = v; return o; }
