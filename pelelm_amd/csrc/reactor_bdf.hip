@@ -34,14 +34,17 @@
 #ifndef PELE_CONSUME_WAVES
 #define PELE_CONSUME_WAVES 3
 #endif
+// k_setup: two waves per SIMD up to 32 unknowns, one above (the 53-species Jacobian
+// measured 7 % faster with the larger register file)
 #ifndef PELE_SETUP_WAVES
-#define PELE_SETUP_WAVES 2
+#define PELE_SETUP_WAVES(N1) ((N1) <= 32 ? 2 : 1)
 #endif
-// k_eval: two waves per SIMD (256 VGPRs) for the mechanisms up to 32 unknowns, one
-// (VGPRs plus accumulation registers for spills) above; measured on drm19 and the
+// k_eval: one wave per SIMD (256 VGPRs plus the accumulation registers, which take the
+// spills) except for the smallest mechanisms, whose right-hand side fits in 128
+// registers; measured on drm19 (spill-free at one wave: 76 -> 68 ms on 128^3) and the
 // 53-species mechanism
 #ifndef PELE_EVAL_WAVES
-#define PELE_EVAL_WAVES(N1) ((N1) <= 32 ? 2 : 1)
+#define PELE_EVAL_WAVES(N1) ((N1) <= 16 ? 4 : 1)
 #endif
 #ifndef PELE_WAVES_PER_EU
 #define PELE_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
@@ -1169,7 +1172,7 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_EVAL_WAVES(M::N1)) 
 
 // Jacobian (when stale) and LU of I - gamma J for the cells starting a Newton attempt
 template <class M>
-__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_SETUP_WAVES) k_setup(const ReactArgs a, const int* __restrict__ list,
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_SETUP_WAVES(M::N1)) k_setup(const ReactArgs a, const int* __restrict__ list,
                                               const unsigned int* __restrict__ count) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
