@@ -238,3 +238,36 @@ def test_auto_schedule_tunes_once_per_box_size():
     assert all(t > 0 for t in tunes[0]) and tunes[0] == tunes[1]   # tuned on the first call only
     print(f"auto schedule, 4096 LiDryer cells: plain {tunes[0][0]:.2f} ms, queue {tunes[0][1]:.2f} ms, "
           f"plain+graphs {tunes[0][2]:.2f} ms, queue+graphs {tunes[0][3]:.2f} ms")
+
+
+def test_typical_values_scale_the_absolute_tolerances():
+    """peleLM.use_typ_vals_chem = 1 with PeleLM's typical values
+    (update_typical_values_chem, Source/PeleLM.cpp:1497-1516: max(typical_Y_val_min,
+    typY * typrho * 1e-3) per species, typT for T): component i's absolute tolerance is
+    ode.atol * typ[i], checked against the oracle run with that tolerance vector."""
+    n = 2048
+    d = cells.generate_workload("drm19_64", ncell=n)
+    N = d["rY"].shape[0]
+    rho = d["rY"].sum(axis=0)
+    typ_rho = rho.max() * 1e3                       # MKS, as PeleLM's typical values are
+    typ_Y = (d["rY"] / rho).max(axis=1)
+    typ = [max(1e-10, typ_Y[k] * typ_rho * 1e-3) for k in range(N)] + [float(d["T"].max())]
+    atol = 1e-8
+    ref = O.Mech("drm19").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6,
+                                atol_vec=np.array(typ) * atol, stats=True)
+    plain = O.Mech("drm19").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=atol, stats=True)
+    assert ref["nfail"] == 0
+    # the vector changes the integration (the scalar tolerance takes other steps)
+    assert not np.array_equal(ref["FC"], plain["FC"])
+    r = _mk("drm19", ode__rtol=1e-6, ode__atol=atol, peleLM__use_typ_vals_chem=1)
+    r.set_typ_vals_ode(typ)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    assert r.stats()["nfail"] == 0
+    r.close()
+    tol = 1e-6 * np.abs(ref["rY"]) + atol * np.array(typ[:N])[:, None]
+    assert (np.abs(rY.cpu().numpy() - ref["rY"]) / tol).max() < 10.0
+    assert np.abs(T.cpu().numpy() - ref["T"]).max() < 1e-2
+    assert np.mean(FC.cpu().numpy() == ref["FC"]) > 0.9
