@@ -333,19 +333,12 @@ template <class M>
 struct Lane {
   static constexpr int N = M::NS, N1 = M::N1;
   using S = Slots<N1>;
-  WS ws;
-  IWS iws;
-  CM mat;
-  const double* xs;   // this trip's Newton solution (matrix region, or its LDS copy)
-  int qmax;
-  double rtol;
-  long mxstep;
-  double tout;
+  // -- persistent state: the words stored in the LANE slots between kernels --
   // history / step control
   double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, delp, acnrm, saved_tq5;
   double tq[6], l[LMAX], tau[LMAX + 1];
   int q, qprime, L, qwait;
-  long nst, nfe, nje, nsetups, nni, ncfn, netf, nstlp, nstlj, nstloc;
+  int nst, nfe, nje, nsetups, nni, ncfn, netf, nstlp, nstlj, nstloc;
   bool jcur;
   // current attempt
   double saved_t, dsm;
@@ -361,14 +354,24 @@ struct Lane {
   // evaluation started an iteration-matrix set-up whose factorisation is to be consumed
   double rE0, fH;
   int setup_pending;
-  // linear algebra: lu_mode / ir_sweeps come from the call; lu64 = 1 once refinement of
-  // FP32 factors has failed for this cell (its later factorisations are FP64)
-  int lu_mode, ir_sweeps, lu64;
-  long nir, nlufb;
-  // learned predictor: use_pred comes from the call; pred_state 0 none, 1 requested for
-  // this attempt (y_n saved in TMP), 2 applied (acor holds prediction - zn0), 3 declined
-  int use_pred, pred_state;
-  long npred, npredfb;
+  // linear algebra: lu64 = 1 once refinement of FP32 factors has failed for this cell
+  // (its later factorisations are FP64)
+  int lu64;
+  int nir, nlufb;
+  // learned predictor: pred_state 0 none, 1 requested for this attempt (y_n saved in
+  // TMP), 2 applied (acor holds prediction - zn0), 3 declined
+  int pred_state;
+  int npred, npredfb;
+  // -- set by load_lane from the call and the cell's addresses, not stored --
+  WS ws;
+  IWS iws;
+  CM mat;
+  const double* xs;   // this trip's Newton solution (matrix region, or its LDS copy)
+  int qmax;
+  double rtol;
+  long mxstep;
+  double tout;
+  int lu_mode, ir_sweeps, use_pred;
 
   __device__ __forceinline__ double& zn(int j, int i) const { return ws[S::ZN + j * N1 + i]; }
 
@@ -955,16 +958,23 @@ __device__ __forceinline__ CM cell_mat(double* mat, long c) {
 // Lane<M> object copied word by word).  The copy lives in scratch memory while a kernel
 // runs: on the step kernel that measured 1.4x faster than holding the whole state in
 // registers, which costs occupancy.
+// words of Lane<M> kept between kernels: the persistent members, which precede `ws`
+template <class M>
+__host__ __device__ constexpr int lane_words() {
+  static_assert(offsetof(Lane<M>, ws) % 8 == 0, "the persistent lane state is whole words");
+  static_assert(offsetof(Lane<M>, ws) / 8 <= Slots<M::N1>::LANE_WORDS, "lane state does not fit its slots");
+  return (int)(offsetof(Lane<M>, ws) / 8);
+}
+
 template <class M>
 __device__ __forceinline__ void load_lane(Lane<M>& st, const ReactArgs& a, long c) {
   using S = Slots<M::N1>;
-  constexpr int NW = (int)(sizeof(Lane<M>) / 8);
-  static_assert(sizeof(Lane<M>) % 8 == 0 && NW <= S::LANE_WORDS, "lane state does not fit its slots");
+  constexpr int NW = lane_words<M>();
   const WS w = cell_ws<M>(a.ws, c);
   double tmp[NW];
 #pragma unroll
   for (int i = 0; i < NW; ++i) tmp[i] = w[S::LANE + i];
-  __builtin_memcpy(&st, tmp, sizeof(Lane<M>));
+  __builtin_memcpy(&st, tmp, NW * 8);
   st.ws = w;
   st.iws = cell_iws<M>(a.iws, c);
   st.mat = cell_mat<M>(a.mat, c);
@@ -980,9 +990,9 @@ __device__ __forceinline__ void load_lane(Lane<M>& st, const ReactArgs& a, long 
 template <class M>
 __device__ __forceinline__ void store_lane(const Lane<M>& st, const WS& w) {
   using S = Slots<M::N1>;
-  constexpr int NW = (int)(sizeof(Lane<M>) / 8);
+  constexpr int NW = lane_words<M>();
   double tmp[NW];
-  __builtin_memcpy(tmp, &st, sizeof(Lane<M>));
+  __builtin_memcpy(tmp, &st, NW * 8);
 #pragma unroll
   for (int i = 0; i < NW; ++i) w[S::LANE + i] = tmp[i];
 }
