@@ -335,13 +335,13 @@ struct Lane {
   using S = Slots<N1>;
   // -- persistent state: the words stored in the LANE slots between kernels --
   // history / step control
-  double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, delp, acnrm, saved_tq5;
+  double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, delp, saved_tq5;
   double tq[6], l[LMAX], tau[LMAX + 1];
   int q, qprime, L, qwait;
   int nst, nfe, nje, nsetups, nni, ncfn, netf, nstlp, nstlj, nstloc;
   bool jcur;
   // current attempt
-  double saved_t, dsm;
+  double saved_t;
   int ncf, nef, nflag, m, convfail;
   bool callSetup;
   // initial step search
@@ -372,6 +372,9 @@ struct Lane {
   long mxstep;
   double tout;
   int lu_mode, ir_sweeps, use_pred;
+  // the converged correction's norm and the error test's measure, used in the trip that
+  // computes them
+  double acnrm, dsm;
 
   __device__ __forceinline__ double& zn(int j, int i) const { return ws[S::ZN + j * N1 + i]; }
 
