@@ -333,26 +333,26 @@ template <class M>
 struct Lane {
   static constexpr int N = M::NS, N1 = M::N1;
   using S = Slots<N1>;
-  // -- persistent state: the words stored in the LANE slots between kernels --
+  // -- persistent state: the words stored in the LANE slots between kernels (8-byte
+  // members first, then 4-byte ones, then flags, so that no word is padding) --
   // history / step control
   double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, delp, saved_tq5;
   double tq[6], l[LMAX], tau[LMAX + 1];
-  int q, qprime, L, qwait;
-  int nst, nfe, nje, nsetups, nni, ncfn, netf, nstlp, nstlj, nstloc;
-  bool jcur;
   // current attempt
   double saved_t;
-  int ncf, nef, nflag, m, convfail;
-  bool callSetup;
   // initial step search
   double hg, hub, hlb, hnew;
-  int count1;
   // evaluation request for the next trip: y = zn0 + a1 * zn1 + a2 * acor at t = teval
   double a1, a2, teval;
-  int phase, status;
-  // cell constants: (rho h)_0 and its forcing; setup_pending = 1 when this trip's
-  // evaluation started an iteration-matrix set-up whose factorisation is to be consumed
+  // cell constants: (rho h)_0 and its forcing
   double rE0, fH;
+  int q, qprime, L, qwait;
+  int nst, nfe, nje, nsetups, nni, ncfn, netf, nstlp, nstlj, nstloc;
+  int ncf, nef, nflag, m, convfail;
+  int count1;
+  int phase, status;
+  // setup_pending = 1 when this trip's evaluation started an iteration-matrix set-up
+  // whose factorisation is to be consumed
   int setup_pending;
   // linear algebra: lu64 = 1 once refinement of FP32 factors has failed for this cell
   // (its later factorisations are FP64)
@@ -362,6 +362,7 @@ struct Lane {
   // TMP), 2 applied (acor holds prediction - zn0), 3 declined
   int pred_state;
   int npred, npredfb;
+  bool jcur, callSetup;
   // -- set by load_lane from the call and the cell's addresses, not stored --
   WS ws;
   IWS iws;
