@@ -1719,9 +1719,13 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_co
     if (!a.lu_lane) st.xs = &sX[threadIdx.x * N1];
     {
       ++st.nfe;
+      // the right-hand side at this trip's point is read only by the phases that use it
+      // (a Newton iteration consumes the solve's result instead)
       double f[N1];
+      auto load_f = [&]() {
 #pragma unroll
-      for (int i = 0; i < N1; ++i) f[i] = w[S::F + i];
+        for (int i = 0; i < N1; ++i) f[i] = w[S::F + i];
+      };
       bool reached = false;
       bool proceed = true;
       if (st.setup_pending) proceed = st.after_setup((int)w[S::SING]);
@@ -1729,16 +1733,18 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_co
         switch (st.phase) {
           case PH_INIT_F0:
 #pragma unroll
-            for (int i = 0; i < N1; ++i) st.zn(1, i) = f[i];
+            for (int i = 0; i < N1; ++i) st.zn(1, i) = w[S::F + i];
             st.after_f0();
             break;
           case PH_HIN:
+            load_f();
             st.after_hin(f);
             break;
           case PH_NEWTON:
             reached = st.iterate();
             break;
           case PH_RESTART_F:
+            load_f();
             st.restart_f(f);
             break;
           default:
