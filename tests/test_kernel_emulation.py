@@ -7,6 +7,7 @@ The device code follows the same algorithm as the oracle, so every cell must tak
 same number of right-hand-side evaluations and agree to rounding.
 """
 import ctypes as C
+import glob
 import os
 import subprocess
 
@@ -23,6 +24,7 @@ SRC = [os.path.join(EMU, "emu.cpp"), os.path.join(EMU, "hip", "hip_runtime.h"),
        os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "reactor_bdf.hip"),
        os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "reactor_internal.h"),
        os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "mlp_internal.h")]
+SRC += sorted(glob.glob(os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc", "mech", "*.inc")))
 CLANG = "/opt/rocm/llvm/bin/clang++"   # host compile; the device code uses clang vector types
 
 
