@@ -1800,7 +1800,9 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_co
     if (__lane_id() == 0 && b) atomicAdd(next_count, (unsigned int)__popcll(b));
     wave_append(want_setup, (int)c, snext, snext_count);
   }
-  if (a.counters) {
+  // the statistics change only when a cell of this wave finished in this trip (a
+  // wave-uniform test: most waves skip the fourteen reductions)
+  if (a.counters && __ballot(t_cells != 0) != 0ull) {
     wave_add(&a.counters[CNT_CELLS], t_cells);
     wave_add(&a.counters[CNT_NST], t_nst);
     wave_add(&a.counters[CNT_NFE], t_nfe);
