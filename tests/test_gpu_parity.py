@@ -271,3 +271,25 @@ def test_typical_values_scale_the_absolute_tolerances():
     assert (np.abs(rY.cpu().numpy() - ref["rY"]) / tol).max() < 10.0
     assert np.abs(T.cpu().numpy() - ref["T"]).max() < 1e-2
     assert np.mean(FC.cpu().numpy() == ref["FC"]) > 0.9
+
+
+@pytest.mark.parametrize("wl,mech", [("drm19_64", "drm19"), ("dodecane53_64_mix", "dodecane53")])
+@pytest.mark.parametrize("lu", ["fp64", "fp32_ir"])
+def test_one_lane_linear_algebra_matches_oracle(wl, mech, lu):
+    """mi.lu_form = lane: one lane per cell factors and solves (the kernels the CPU
+    emulation runs), in FP64 and with FP32 factors plus refinement, in the reactor."""
+    n = 1024
+    d = cells.generate_workload(wl, ncell=n)
+    ref = O.Mech(mech).react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    r = _mk(mech, ode__rtol=1e-6, ode__atol=1e-10, mi__lu_form="lane", mi__lu=lu)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    assert st["nfail"] == 0 and st["ncells"] == n
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    assert (np.abs(rY.cpu().numpy() - ref["rY"]) / tol).max() < 10.0
+    assert np.abs(T.cpu().numpy() - ref["T"]).max() < 1e-2
+    assert np.mean(FC.cpu().numpy() == ref["FC"]) > 0.9
