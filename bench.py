@@ -60,7 +60,7 @@ def rhs_ops(mech: str):
     return fixed, per_iter, trans
 
 
-LANE_WORDS = 57   # 8-byte words of the integrator's stored lane state (independent of the mechanism)
+LANE_WORDS = 53   # 8-byte words of the integrator's stored lane state (independent of the mechanism)
 
 
 def rhs_bytes(N: int) -> int:

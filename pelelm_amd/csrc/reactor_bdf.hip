@@ -337,11 +337,19 @@ struct Lane {
   // members first, then 4-byte ones, then flags, so that no word is padding) --
   // history / step control
   double tn, h, hprime, hscale, eta, etamax, gamma, gammap, gamrat, rl1, crate, delp, saved_tq5;
-  double tq[6], l[LMAX], tau[LMAX + 1];
+  double tq[6];
+  // the BDF coefficients l share their words with the initial step search, which
+  // ends before the first attempt computes them (set_coeffs writes l before any read)
+  union {
+    double l[LMAX];
+    struct {
+      double hg, hub, hlb, hnew;
+      int count1;
+    };
+  };
+  double tau[LMAX + 1];
   // current attempt
   double saved_t;
-  // initial step search
-  double hg, hub, hlb, hnew;
   // evaluation request for the next trip: y = zn0 + a1 * zn1 + a2 * acor at t = teval
   double a1, a2, teval;
   // cell constants: (rho h)_0 and its forcing
@@ -349,7 +357,6 @@ struct Lane {
   int q, qprime, L, qwait;
   int nst, nfe, nje, nsetups, nni, ncfn, netf, nstlp, nstlj, nstloc;
   int ncf, nef, nflag, m, convfail;
-  int count1;
   int phase, status;
   // setup_pending = 1 when this trip's evaluation started an iteration-matrix set-up
   // whose factorisation is to be consumed
