@@ -6,8 +6,9 @@
  * `peleLM.chem_integrator` name, initialised as an h-reactor, given typical values for
  * the absolute tolerances, asked to react boxes, closed at teardown.  Every entry point
  * takes plain pointers and sizes; device arrays are HIP device pointers and all work is
- * enqueued on the caller's HIP stream (asynchronous, like the reference's GPU path,
- * which synchronises right after the call, Source/PeleLM.cpp:5773-5775).
+ * enqueued on the caller's HIP stream.  The host-driven schedules read the work-list
+ * counts back once per trip, so react returns with the integration finished (the
+ * reference's caller synchronises right after the call anyway, Source/PeleLM.cpp:5773-5775).
  *
  * Units are CGS in and out, as the reference converts before and after the call
  * (Source/PeleLM.cpp:5737-5746, 5764-5771).
@@ -98,8 +99,13 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
      issues every trip, default | K even: K trips per captured HIP graph), mi.lu (fp64 |
      fp32_ir: FP32 factors with FP64-residual refinement), mi.ir_sweeps (3), mi.lu_form
      (coop: 32 lanes per cell, a whole wave above 32 unknowns, default | lane: one lane per
-     cell), mi.on_fail
-     (continue | abort), mi.profile (0 | 1: per-kernel event timing). */
+     cell), mi.ir_tol (1e-4: a refinement sweep whose correction has WRMS norm at most this,
+     in error-weight units, ends the refinement; a solve that does not get there in
+     mi.ir_sweeps sweeps makes its cell refactor in FP64), mi.max_trips (0: 3 x
+     cvode.max_num_steps + 100; cells still integrating after this many trips fail),
+     mi.on_fail (abort: react returns MI_ECELLFAIL when a cell failed, default | continue:
+     react returns MI_OK, failed cells carry FC < 0), mi.profile (0 | 1: per-kernel event
+     timing). */
 int mi_chem_set(void* reactor, const char* key, const char* value);
 
 /* ReactorBase::set_typ_vals_ode -- Source/PeleLM.cpp:1497-1516: n = nspec + 1 typical
@@ -111,7 +117,9 @@ int mi_chem_set_typ_vals(void* reactor, const double* typ, int n);
    Source/PeleLM.cpp:5752-5758.  rY holds nspec components, rYsrc nspec, T/rE/rEsrc/FC
    one each; cells of [lo, hi] (inclusive) with mask >= 0 are advanced from time to
    time + dt in place.  On return *time += dt.  FC receives the number of RHS
-   evaluations per cell (negative for a failed cell). */
+   evaluations per cell; a failed cell keeps its input state and gets FC = -(its RHS
+   count).  Returns MI_ECELLFAIL when a cell failed under mi.on_fail = abort (the
+   default), MI_OK under mi.on_fail = continue. */
 int mi_chem_react_box(void* reactor, const int lo[3], const int hi[3], mi_real4 rY, mi_real4 rYsrc,
                       mi_real4 T, mi_real4 rE, mi_real4 rEsrc, mi_real4 FC, mi_int4 mask, double* dt,
                       double* time, void* hip_stream);
@@ -152,7 +160,8 @@ int mi_chem_advance_box(void* reactor, const int lo[3], const int hi[3], mi_real
 int mi_chem_rhoh_to_temp_box(void* reactor, const int lo[3], const int hi[3], mi_real4 rho, mi_real4 rhoY,
                              mi_real4 rhoH, mi_real4 T, void* hip_stream);
 
-/* Statistics of the last react call (synchronises the stream of that call). */
+/* Statistics of the last react call (synchronises the stream of that call); returns
+   MI_OK whether or not cells failed (nfail counts them). */
 int mi_chem_get_stats(void* reactor, mi_chem_stats* out);
 
 /* Number of species of the selected mechanism, or < 0. */

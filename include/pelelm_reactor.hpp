@@ -14,8 +14,10 @@
 // j, k, component), and a driver built with AMReX converts with `mi::view(array4)`.
 //
 // Errors: the C ABI returns negative codes; this adapter throws std::runtime_error with
-// the reactor's message, except that react() returns the code (the reference's caller
-// ignores it, Source/PeleLM.cpp:5752) when mi.on_fail = continue.
+// the reactor's message.  A cell that fails to integrate throws from react() under
+// mi.on_fail = abort (the default: the reference's integrator aborts inside react, and its
+// caller ignores the return value, Source/PeleLM.cpp:5752); under mi.on_fail = continue
+// react() returns 0 and the failed cells carry FC < 0.
 #pragma once
 
 #include <memory>
@@ -109,8 +111,8 @@ class ReactorBase {
   }
 
   // Advance every cell of `box` with mask >= 0 from `time` over `dt` on `stream`
-  // (a hipStream_t, or nullptr for the null stream).  Returns 0, or MI_ECELLFAIL when a
-  // cell failed and mi.on_fail = continue.
+  // (a hipStream_t, or nullptr for the null stream).  Returns 0; throws on any error,
+  // including a failed cell under mi.on_fail = abort (the default).
   template <class R4, class I4>
   int react(const Box& box, const R4& rY, const R4& rYsrc, const R4& T, const R4& rE, const R4& rEsrc, const R4& FC,
             const I4& mask, double& dt, double& time, void* stream = nullptr) {
@@ -118,8 +120,7 @@ class ReactorBase {
     int hi[3] = {box.hi[0], box.hi[1], box.hi[2]};
     const int rc = mi_chem_react_box(h_, lo, hi, view(rY), view(rYsrc), view(T), view(rE), view(rEsrc), view(FC),
                                      view_int(mask), &dt, &time, stream);
-    if (rc != MI_OK && rc != MI_ECELLFAIL) check(rc);
-    return rc;
+    return check(rc);
   }
 
   mi_chem_stats stats() {

@@ -229,11 +229,13 @@ static void rhs_at_T(const orc_mech* m, const S* y, S T, const double* fY, doubl
   for (int k = 0; k < N; ++k) C[k] = y[k] / m->W[k];
   wdot_t<S>(m, C, T, wd, (S*)nullptr);
   thermo_t<S>(m, T, cpR, hRT, nullptr);
+  // energy closure from rho h = sum_k rhoY_k h_k(T): rho cp dT/dt = F_h - sum_k h_k d(rhoY_k)/dt,
+  // where d(rhoY_k)/dt = W_k wdot_k + F_Y,k (the species forcing carries its enthalpy)
   S rhocp = 0.0, hw = 0.0;
   for (int k = 0; k < N; ++k) {
     yd[k] = wd[k] * m->W[k] + (fY ? fY[k] : 0.0);
     rhocp += y[k] * cpR[k] / m->W[k];
-    hw += wd[k] * hRT[k];
+    hw += (wd[k] + (fY ? fY[k] / m->W[k] : 0.0)) * hRT[k];
   }
   rhocp *= m->Ru;
   hw *= m->Ru * T;
@@ -383,11 +385,14 @@ void orc_jac(const orc_mech* m, const double* y, const double* fY, double fH, do
     for (int i = 0; i < r.np; ++i) scatter(r.pi[i], double(r.pc[i]));
   }
   // assemble J of rhs_at_T
+  // molar species source wdot_k + F_Y,k / W_k (what the energy equation weighs by h_k)
+  double src[ORC_NMAX];
+  for (int k = 0; k < N; ++k) src[k] = wd[k] + (fY ? fY[k] / m->W[k] : 0.0);
   double rhocp = 0.0, drhocp = 0.0, hw = 0.0;
   for (int k = 0; k < N; ++k) {
     rhocp += y[k] * cpR[k] / m->W[k];
     drhocp += y[k] * dcp[k] / m->W[k];
-    hw += wd[k] * hRT[k];
+    hw += src[k] * hRT[k];
   }
   rhocp *= m->Ru;
   drhocp *= m->Ru;
@@ -397,7 +402,8 @@ void orc_jac(const orc_mech* m, const double* y, const double* fY, double fH, do
     for (int k = 0; k < N; ++k) J[i * n + k] = m->W[i] / m->W[k] * dWdC[i * N + k];
     J[i * n + N] = m->W[i] * dWdT[i];
   }
-  // T row: fT = (fH - sum_i H_i wdot_i)/rhocp with H_i = hRT_i R T (molar)
+  // T row: fT = (fH - sum_i H_i (wdot_i + F_Y,i / W_i))/rhocp with H_i = hRT_i R T (molar);
+  // the forcing is constant, so only wdot_i depends on rhoY_k
   for (int k = 0; k < N; ++k) {
     double s = 0.0;
     for (int i = 0; i < N; ++i) s += hRT[i] * dWdC[i * N + k];
@@ -407,7 +413,7 @@ void orc_jac(const orc_mech* m, const double* y, const double* fY, double fH, do
   }
   {
     double s = 0.0;
-    for (int i = 0; i < N; ++i) s += cpR[i] * wd[i] + hRT[i] * T * dWdT[i];
+    for (int i = 0; i < N; ++i) s += cpR[i] * src[i] + hRT[i] * T * dWdT[i];
     s *= m->Ru;
     J[N * n + N] = (-s - fT * drhocp) / rhocp;
   }

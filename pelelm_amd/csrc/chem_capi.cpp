@@ -38,6 +38,8 @@ struct Reactor {
   int schedule = SCHED_PLAIN;     // measured fastest on every workload here
   int lu_mode = LU_FP64;
   int ir_sweeps = 3;
+  double ir_tol = 1e-4;         // refinement acceptance (WRMS of the correction, error-weight units)
+  long max_trips = 0;           // 0: 3 x cvode.max_num_steps + 100 trips
   int lu_lane = 0;
   void* predictor = nullptr;    // attached network (mi_chem_attach_predictor), not owned
   int graph_trips = 0;          // 0: host-driven trips; K (even): graphs of K trips
@@ -49,7 +51,10 @@ struct Reactor {
   };
   std::map<long, Tuned> tuned;
   int last_schedule = SCHED_COMPACT, last_graph = 0;
-  bool on_fail_continue = true;
+  // a cell that exhausts every fallback: abort (default) makes react return MI_ECELLFAIL,
+  // as the reference's integrator aborts inside react; continue returns MI_OK, the
+  // failed cells carrying FC < 0 and counted in the statistics
+  bool on_fail_continue = false;
   bool profile = false;
   KernelProf prof{};
   // device state
@@ -219,6 +224,7 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   a.mxstep = r->max_steps;
   a.lu_mode = r->lu_mode;
   a.ir_sweeps = r->ir_sweeps;
+  a.ir_tol = r->ir_tol;
   a.lu_lane = r->lu_lane;
   a.compact = (r->schedule == SCHED_COMPACT) ? 1 : 0;
   int graph_trips = r->graph_trips;
@@ -243,7 +249,7 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   a.counts = r->counts;
   a.counters = r->counters;
   // every step takes at least one trip; 3 per step bounds retries generously
-  const long max_trips = 3 * r->max_steps + 100;
+  const long max_trips = r->max_trips > 0 ? r->max_trips : 3 * r->max_steps + 100;
   if (graph_trips > 0 || r->schedule == SCHED_AUTO) {
     if (!r->capture_stream) HIPCHK(r, hipStreamCreateWithFlags(&r->capture_stream, hipStreamNonBlocking));
   }
@@ -276,6 +282,15 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   r->last_stream = stream;
   r->have_stats = true;
   *time += *dt;
+  if (!r->on_fail_continue && !r->null_reactor) {
+    // the trips have already synchronised the stream once per trip: one more read of the
+    // failure counter decides the return code
+    unsigned long long nfail = 0;
+    HIPCHK(r, hipMemcpyAsync(&nfail, r->counters + CNT_NFAIL, sizeof(nfail), hipMemcpyDeviceToHost, stream));
+    HIPCHK(r, hipStreamSynchronize(stream));
+    if (nfail > 0)
+      return fail(r, MI_ECELLFAIL, std::to_string(nfail) + " cell(s) failed to integrate (FC < 0 marks them)");
+  }
   return MI_OK;
 }
 
@@ -391,6 +406,12 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
   } else if (k == "mi.ir_sweeps") {
     if (!parse_long(value, &l) || l < 0 || l > 10) return fail(r, MI_EINVAL, "mi.ir_sweeps must be 0..10");
     r->ir_sweeps = (int)l;
+  } else if (k == "mi.ir_tol") {
+    if (!parse_double(value, &d) || d < 0.0) return fail(r, MI_EINVAL, "mi.ir_tol must be a number >= 0");
+    r->ir_tol = d;
+  } else if (k == "mi.max_trips") {
+    if (!parse_long(value, &l) || l < 0) return fail(r, MI_EINVAL, "mi.max_trips must be >= 0 (0: automatic)");
+    r->max_trips = l;
   } else if (k == "mi.profile") {
     if (!parse_long(value, &l) || (l != 0 && l != 1)) return fail(r, MI_EINVAL, "mi.profile must be 0 or 1");
     r->profile = (l == 1);
@@ -594,7 +615,6 @@ int mi_chem_get_stats(void* reactor, mi_chem_stats* out) {
   auto it = r->tuned.find(r->last_ncell);
   if (it != r->tuned.end())
     for (int v = 0; v < 4; ++v) out->tune_ms[v] = it->second.ms[v];
-  if (out->nfail > 0 && !r->on_fail_continue) return fail(r, MI_ECELLFAIL, "cells failed to integrate");
   return MI_OK;
 }
 

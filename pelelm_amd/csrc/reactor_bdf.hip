@@ -16,9 +16,8 @@
 // per-lane history (Nordsieck array, saved Jacobian, LU factors, weights) lives in a
 // wave-tiled workspace -- element (slot, lane) of wave w at ws[(w*nslot+slot)*64+lane]
 // -- so every access of one slot by a wave is one contiguous 512-byte line.
-// Two schedules: one thread per cell (plain), or a persistent grid whose waves claim
-// chunks of cells from a device counter and refill a lane as soon as its cell is done
-// (queue), which keeps lanes busy when cell stiffness is uneven.
+// Trips of bounded kernels (evaluation, set-up, LU, solve, step logic) issued by the
+// host, with a fixed cell-to-lane mapping (plain) or compacted work lists (queue).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -66,9 +65,6 @@ constexpr int SMALL_NST = 10, MXNCF = 10, MXNEF = 7, MXNEF1 = 3, SMALL_NEF = 2, 
 constexpr int MAXCOR = 3, MSBP = 20, MSBJ = 51, HIN_ITERS = 4;
 constexpr int LMAX = 6;
 constexpr int HY2T_MAXIT = 50;
-// iterative refinement of FP32 factors: a sweep whose correction has WRMS norm (in
-// units of the error weights) at or below IR_TOL ends the refinement
-constexpr double IR_TOL = 1e-4;
 // Newton convergence threshold factor for attempts that start from a learned iterate
 constexpr double PRED_NLS_TIGHTEN = 0.01;
 constexpr double HY2T_RTOL = 1e-10;
@@ -187,12 +183,15 @@ __device__ __forceinline__ void rhs(const double* __restrict__ y, const double t
   M::wdot(C, T, logT, invT, g, wd);
   double cpR[N], hRT[N];
   M::thermo_cph(T, cpR, hRT);
+  // energy closure from rho h = sum_k rhoY_k h_k(T): rho cp dT/dt = F_h - sum_k h_k d(rhoY_k)/dt
+  // with d(rhoY_k)/dt = W_k wdot_k + F_Y,k (the species forcing carries its enthalpy)
   double rhocp = 0.0, hw = 0.0;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    f[k] = wd[k] * M::mw(k) + ws[S::FY + k];
+    const double fy = ws[S::FY + k];
+    f[k] = wd[k] * M::mw(k) + fy;
     rhocp += y[k] * cpR[k] * M::imw(k);
-    hw += wd[k] * hRT[k];
+    hw += (wd[k] + fy * M::imw(k)) * hRT[k];
   }
   f[N] = (fH - hw * M::RU * T) / (rhocp * M::RU);
 }
@@ -214,7 +213,8 @@ __device__ __noinline__ void jacobian(const double* __restrict__ y, const double
 #pragma unroll
   for (int k = 0; k < N; ++k) rhocp += y[k] * cpR[k] * M::imw(k);
   rhocp *= M::RU;
-  // T row: d fT / d rhoY_k = (-R T sum_i hRT_i/W_i J_ik - fT cp_k) / rhocp
+  // T row: fT = (F_h - R T sum_i hRT_i (wdot_i + F_Y,i / W_i)) / rhocp (fT is passed in);
+  // d fT / d rhoY_k = (-R T sum_i hRT_i/W_i J_ik - fT cp_k) / rhocp, the forcing being constant
   double s = 0.0;
   M::jacobian(C, T, logT, invT, g, hRT, wd, dwdT,
               [&](int i, int k, double v) {
@@ -231,7 +231,7 @@ __device__ __noinline__ void jacobian(const double* __restrict__ y, const double
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     drhocp += y[k] * dcp[k] * M::imw(k);
-    sTT += cpR[k] * wd[k] + hRT[k] * T * dwdT[k];
+    sTT += cpR[k] * (wd[k] + ws[S::FY + k] * M::imw(k)) + hRT[k] * T * dwdT[k];
     ws[S::JS + k * N1 + N] = M::mw(k) * dwdT[k];
   }
   drhocp *= M::RU;
@@ -937,6 +937,11 @@ struct Lane {
     for (int j = 0; j < LMAX; ++j)
 #pragma unroll
       for (int i = 0; i < N1; ++i) zn(j, i) = (j == 0) ? y0[i] : 0.0;
+    // the first evaluations (f0 and the initial-step search) form zn0 + a1 zn1 + a2 acor
+    // with a2 = 0: acor must hold numbers, since 0 x NaN is NaN in whatever the
+    // workspace held before
+#pragma unroll
+    for (int i = 0; i < N1; ++i) ws[S::ACOR + i] = 0.0;
     ewtset();
     status = ST_OK;
     phase = PH_INIT_F0;
@@ -1549,7 +1554,7 @@ __device__ __forceinline__ bool solve_due(const WS& w) {
 // Newton solves, one lane per cell (the form that runs when the kernels execute
 // serially on a CPU): x = M^-1 B with FP64 factors, or with FP32 factors and up to
 // ir_sweeps refinement sweeps r = B - M x (M = I - gammap J from the FP64 Jacobian),
-// x += M^-1 r, stopping once the correction's WRMS norm is at most IR_TOL.
+// x += M^-1 r, stopping once the correction's WRMS norm is at most a.ir_tol.
 template <class M>
 __global__ void __launch_bounds__(64) k_solve_lane(const ReactArgs a, const int* __restrict__ list,
                                                    const unsigned int* __restrict__ count) {
@@ -1590,7 +1595,7 @@ __global__ void __launch_bounds__(64) k_solve_lane(const ReactArgs a, const int*
       const double q = d * w[S::EWT + i];
       nrm += q * q;
     }
-    ok = (sqrt(nrm / N1) <= IR_TOL);   // false for a NaN norm
+    ok = (sqrt(nrm / N1) <= a.ir_tol);   // false for a NaN norm
   }
   mat[MS::IRS] = (double)sweeps;
   mat[MS::IRF] = ok ? 0.0 : 1.0;
@@ -1671,7 +1676,7 @@ __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* _
       q *= q;
 #pragma unroll
       for (int off = G / 2; off > 0; off >>= 1) q += __shfl_xor(q, off, G);
-      ok = (sqrt(q / N1) <= IR_TOL);
+      ok = (sqrt(q / N1) <= a.ir_tol);
     }
     if (work && r == 0) {
       mat[MS::IRS] = (double)sweeps;

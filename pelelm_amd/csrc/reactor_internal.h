@@ -43,6 +43,8 @@ struct ReactArgs {
   long mxstep;
   int lu_mode;
   int ir_sweeps;
+  double ir_tol;                         // a refinement sweep whose correction has WRMS norm (in
+                                         // units of the error weights) <= ir_tol ends the refinement
   int lu_lane;                           // 1: one lane factors a cell (the form that runs when
                                          // the kernels execute serially on a CPU); 0: 32 lanes
   // per-cell workspace, tiled by 64 cells: element (slot s) of cell c at

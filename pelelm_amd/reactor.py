@@ -104,6 +104,11 @@ class ReactorError(RuntimeError):
     pass
 
 
+class CellFailure(ReactorError):
+    """A cell failed to integrate under ``mi.on_fail = abort`` (the default), where the
+    reference's integrator aborts inside ``react``; failed cells carry FC < 0."""
+
+
 class ReactorBase:
     """``ReactorBase`` with the reference's method names (create / init / set_typ_vals_ode /
     react / close), backed by the MI355X reactor."""
@@ -124,7 +129,8 @@ class ReactorBase:
     def _check(self, rc):
         if rc != MI_OK:
             msg = lib().mi_chem_last_error(self._h)
-            raise ReactorError(f"{rc}: {msg.decode() if msg else ''}")
+            err = CellFailure if rc == MI_ECELLFAIL else ReactorError
+            raise err(f"{rc}: {msg.decode() if msg else ''}")
         return rc
 
     def init(self, reactor_type: int = 2, ncells: int = 1):

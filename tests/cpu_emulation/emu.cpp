@@ -9,7 +9,10 @@ thread_local Idx3 threadIdx, blockIdx, blockDim, gridDim;
 
 using namespace pelelm;
 
-static int g_lu_mode = LU_FP64, g_ir_sweeps = 3, g_pred = 0;
+static int g_lu_mode = LU_FP64, g_ir_sweeps = 3, g_pred = 0, g_fill = -1;
+// byte pattern the workspace, lists and matrix region are filled with before a run
+// (-1: left as allocated), so a read of a word the integrator never wrote shows up
+extern "C" void emu_set_fill(int byte) { g_fill = byte; }
 // 1: attach a predictor whose network is all zeros (its iterate is the state at tn,
 // the "previous value" guess) -- the predictor's plumbing without a trained network
 extern "C" void emu_set_pred(int on) { g_pred = on; }
@@ -39,6 +42,12 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   int* iws = new int[(size_t)nw * iws_slots(m) * 64];
   int* lists = new int[6 * (size_t)nw * 64];
   double* mat = new double[(size_t)nw * 64 * mat_stride(m)];
+  if (g_fill >= 0) {
+    std::memset(ws, g_fill, (size_t)nw * ws_slots(m) * 64 * sizeof(double));
+    std::memset(iws, g_fill, (size_t)nw * iws_slots(m) * 64 * sizeof(int));
+    std::memset(lists, g_fill, 6 * (size_t)nw * 64 * sizeof(int));
+    std::memset(mat, g_fill, (size_t)nw * 64 * mat_stride(m) * sizeof(double));
+  }
   unsigned int counts[8] = {0};
   a.ws = ws; a.iws = iws; a.mat = mat;
   a.plist = lists + (size_t)4 * nw * 64;
@@ -52,6 +61,7 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   a.lu_lane = 1;   // lanes run one at a time here, so they cannot cooperate
   a.lu_mode = g_lu_mode;
   a.ir_sweeps = g_ir_sweeps;
+  a.ir_tol = 1e-4;
   a.counters = counters;
   MlpDev md{};
   std::vector<unsigned short> wz(2 * 64 * 128 * 8, 0);

@@ -73,6 +73,7 @@ __global__ void k_jac_lu(const double* y, const double* T, const double* fT, con
   if ((threadIdx.x % 3) != 0) {
     double yy[N1];
     for (int i = 0; i < N1; ++i) yy[i] = y[c * N1 + i];
+    for (int k = 0; k < N1 - 1; ++k) w[S::FY + k] = 0.0;   // no species forcing
     jacobian<M>(yy, T[c], fT[c], w);
     for (int e = 0; e < N1 * N1; ++e) mat[MatSlots<N1>::JS + e] = w[S::JS + e];
     sg = lu_factor<N1, double>(mat, gamma[c]);
@@ -366,6 +367,7 @@ int unit_lu_t(int coop, int fp32, int sweeps, long n, const double* J, const dou
   a.counts = dcount;
   a.lu_mode = fp32 ? LU_FP32_IR : LU_FP64;
   a.ir_sweeps = sweeps;
+  a.ir_tol = 1e-4;
   a.lu_lane = coop ? 0 : 1;
   hipLaunchKernelGGL(k_lu_prep<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, dJ, dg, dp, db, de, ws, mats, n);
   hcounts[0] = (unsigned)n;

@@ -2,8 +2,11 @@
 
 Every call goes through the C ABI (pelelm_amd/reactor.py -> libpelelm_chem.so).
 Tolerances: production rates are deterministic FP64 arithmetic and agree to 1e-11
-relative; integrated states follow the same algorithm at rtol 1e-6, so each component
-must agree within 10 (rtol |y| + atol) and T within 1e-2 K.
+relative.  The FP64 integrations run the oracle's algorithm step for step, so they are
+held to the agreement that identical step sequences give: every component within 1e-2
+of its tolerance (rtol |y| + atol), T within 1e-7 K, and the same RHS count (FC) in at
+least 99 % of the cells.  Forms with FP32 factors are held to the tolerance itself (their
+rounding can move a step decision in a cell, which then differs by O(rtol)).
 """
 import numpy as np
 import pytest
@@ -29,6 +32,19 @@ def _mk(mech, **keys):
 
 def _dev(a, dtype=torch.float64):
     return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def _parity(label, gY, gT, gFC, ref, tol=None, y_frac=1e-2, dT=1e-7, fc=0.99):
+    """The GPU states against the oracle's: max |dY| / tol, max |dT| and the share of
+    cells with the oracle's RHS count, printed and checked against the bounds."""
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10 if tol is None else tol
+    ey = float((np.abs(gY - ref["rY"]) / tol).max())
+    et = float(np.abs(gT - ref["T"]).max())
+    same = float(np.mean(gFC == ref["FC"])) if gFC is not None else 1.0
+    print(f"{label}: max |dY|/tol {ey:.3g}, max |dT| {et:.3g} K, identical FC {same:.4f}")
+    assert ey <= y_frac, (label, ey)
+    assert et <= dT, (label, et)
+    assert same >= fc, (label, same)
 
 
 @pytest.mark.parametrize("wl,mech", [("lidryer_16", "LiDryer"), ("drm19_64", "drm19"),
@@ -72,13 +88,8 @@ def test_react_matches_oracle(wl, mech, n, schedule):
     assert st["nfail"] == 0 and st["ncells"] == n
     assert tm == pytest.approx(d["dt"])
     gY, gT, gE, gFC = rY.cpu().numpy(), T.cpu().numpy(), rE.cpu().numpy(), FC.cpu().numpy()
-    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
-    err = np.abs(gY - ref["rY"]) / tol
-    assert err.max() < 10.0, err.max()
-    assert np.abs(gT - ref["T"]).max() < 1e-2
+    _parity(f"{wl} {schedule}", gY, gT, gFC, ref)
     np.testing.assert_array_equal(gE, ref["rE"])
-    # same algorithm: most cells take exactly the oracle's number of RHS evaluations
-    assert np.mean(gFC == ref["FC"]) > 0.9
     # mass conservation is exact up to rounding
     np.testing.assert_allclose(gY.sum(0), d["rY"].sum(0), rtol=1e-12)
 
@@ -103,9 +114,7 @@ def test_fp32_ir_matches_fp64_and_tight_oracle(wl, mech):
     r.close()
     assert st["nfail"] == 0 and st["nlu_refine"] > 0
     gY, gT = rY.cpu().numpy(), T.cpu().numpy()
-    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
-    assert (np.abs(gY - ref["rY"]) / tol).max() < 10.0
-    assert np.abs(gT - ref["T"]).max() < 1e-2
+    _parity(f"{wl} fp32_ir", gY, gT, FC.cpu().numpy(), ref, y_frac=1.0, dT=1e-3, fc=0.95)
     e_ir = np.abs(gT - tight["T"]).max()
     e_64 = np.abs(ref["T"] - tight["T"]).max()
     print(f"{wl}: max|dT| vs tight oracle: fp32_ir {e_ir:.3g} K, fp64 {e_64:.3g} K; "
@@ -129,11 +138,39 @@ def test_forcing_matches_oracle():
     assert r.stats()["nfail"] == 0
     r.close()
     gY, gT, gE = rY.cpu().numpy(), T.cpu().numpy(), rE.cpu().numpy()
-    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
-    assert (np.abs(gY - ref["rY"]) / tol).max() < 10.0
-    assert np.abs(gT - ref["T"]).max() < 1e-2
+    _parity("drm19_64 forced", gY, gT, FC.cpu().numpy(), ref)
     np.testing.assert_allclose(gE, d["rE"] + d["dt"] * fH, rtol=1e-15)
     np.testing.assert_allclose(gY.sum(0), d["rY"].sum(0) + d["dt"] * fY.sum(0), rtol=1e-12)
+
+
+@pytest.mark.parametrize("wl,mech", [("drm19_64", "drm19"), ("dodecane53_64_mix", "dodecane53")])
+def test_forced_T_stays_on_its_enthalpy(wl, mech):
+    """rho h = sum_k rhoY_k h_k(T): after a forced advance the temperature the kernels
+    integrate must be the one the advanced (rho h, rhoY) define, T_out = HY2T(rhoh_out /
+    rho_out, Y_out), to 1e-6 K at rtol 1e-8 (an energy equation without the enthalpy the
+    species forcing carries misses by about 0.2 K here)."""
+    n = 256
+    d = cells.generate_workload(wl, ncell=n)
+    m = O.Mech(mech)
+    dt = 1e-6
+    rng = np.random.default_rng(11)
+    fY = rng.uniform(-1.0, 1.0, size=d["rY"].shape) * 1e-4 * d["rY"] / dt
+    fH = rng.uniform(-1.0, 1.0, size=n) * 1e-4 * np.abs(d["rE"]) / dt
+    r = _mk(mech, ode__rtol=1e-8, ode__atol=1e-14, cvode__max_num_steps=200000)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, _dev(fY), T, rE, _dev(fH), FC, None, dt, 0.0)
+    torch.cuda.synchronize()
+    assert r.stats()["nfail"] == 0
+    r.close()
+    gY, gT, gE = rY.cpu().numpy(), T.cpu().numpy(), rE.cpu().numpy()
+    rho = gY.sum(axis=0)
+    worst = 0.0
+    for c in range(n):
+        Tc, _ = m.hy2t(gE[c] / rho[c], gY[:, c] / rho[c], gT[c])
+        worst = max(worst, abs(gT[c] - Tc))
+    print(f"{wl}: max |T_out - HY2T(rhoh_out, Y_out)| = {worst:.3g} K under forcing at rtol 1e-8")
+    assert worst <= 1e-6
 
 
 def test_mask_skips_covered_cells():
@@ -154,9 +191,9 @@ def test_mask_skips_covered_cells():
     np.testing.assert_array_equal(gY[:, ::3], d["rY"][:, ::3])
     np.testing.assert_array_equal(gT[::3], d["T"][::3])
     act = mask >= 0
-    tol = 1e-6 * np.abs(ref["rY"][:, act]) + 1e-10
-    assert (np.abs(gY[:, act] - ref["rY"][:, act]) / tol).max() < 10.0
-    assert np.all(gFC[act] > 0)
+    sub = {"rY": ref["rY"][:, act], "T": ref["T"][act], "FC": ref["FC"][act]}
+    _parity("lidryer_16 masked", gY[:, act], gT[act], gFC[act], sub)
+    np.testing.assert_array_equal(gFC[~act], 0.0)
 
 
 def test_react_box_on_array4_views_with_ghost_cells():
@@ -188,9 +225,7 @@ def test_react_box_on_array4_views_with_ghost_cells():
     flat_in = fab[inner].reshape(N + 2, -1)
     ref = O.Mech("drm19").react(flat_in[:N], flat_in[N + 1], flat_in[N], d["dt"], rtol=1e-6, atol=1e-10)
     got = out[inner].reshape(N + 2, -1)
-    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
-    assert (np.abs(got[:N] - ref["rY"]) / tol).max() < 10.0
-    assert np.abs(got[N + 1] - ref["T"]).max() < 1e-2
+    _parity("drm19 Array4 box", got[:N], got[N + 1], None, ref)
 
 
 @pytest.mark.parametrize("wl,mech", [("drm19_128_mix", "drm19"), ("dodecane53_64_mix", "dodecane53")])
@@ -209,10 +244,7 @@ def test_graph_captured_trips_match_oracle(schedule, wl, mech):
     st = r.stats()
     r.close()
     assert st["nfail"] == 0 and st["graph_trips_used"] == 8 and st["ntrips"] % 8 == 0
-    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
-    assert (np.abs(rY.cpu().numpy() - ref["rY"]) / tol).max() < 10.0
-    assert np.abs(T.cpu().numpy() - ref["T"]).max() < 1e-2
-    assert np.mean(FC.cpu().numpy() == ref["FC"]) > 0.9
+    _parity(f"{wl} {schedule} graphs", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)
 
 
 def test_auto_schedule_tunes_once_per_box_size():
@@ -229,8 +261,7 @@ def test_auto_schedule_tunes_once_per_box_size():
         torch.cuda.synchronize()
         st = r.stats()
         assert st["nfail"] == 0
-        assert (np.abs(rY.cpu().numpy() - ref["rY"]) / tol).max() < 10.0
-        assert np.abs(T.cpu().numpy() - ref["T"]).max() < 1e-2
+        _parity(f"lidryer_16 auto call {call}", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)
         tunes.append(list(st["tune_ms"]))
         best = int(np.argmin(st["tune_ms"]))
         assert st["schedule_used"] == best % 2 and st["graph_trips_used"] == (8 if best >= 2 else 0)
@@ -268,9 +299,7 @@ def test_typical_values_scale_the_absolute_tolerances():
     assert r.stats()["nfail"] == 0
     r.close()
     tol = 1e-6 * np.abs(ref["rY"]) + atol * np.array(typ[:N])[:, None]
-    assert (np.abs(rY.cpu().numpy() - ref["rY"]) / tol).max() < 10.0
-    assert np.abs(T.cpu().numpy() - ref["T"]).max() < 1e-2
-    assert np.mean(FC.cpu().numpy() == ref["FC"]) > 0.9
+    _parity("drm19_64 typical values", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref, tol=tol)
 
 
 @pytest.mark.parametrize("wl,mech", [("drm19_64", "drm19"), ("dodecane53_64_mix", "dodecane53")])
@@ -289,7 +318,8 @@ def test_one_lane_linear_algebra_matches_oracle(wl, mech, lu):
     st = r.stats()
     r.close()
     assert st["nfail"] == 0 and st["ncells"] == n
-    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
-    assert (np.abs(rY.cpu().numpy() - ref["rY"]) / tol).max() < 10.0
-    assert np.abs(T.cpu().numpy() - ref["T"]).max() < 1e-2
-    assert np.mean(FC.cpu().numpy() == ref["FC"]) > 0.9
+    if lu == "fp64":
+        _parity(f"{wl} lane fp64", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)
+    else:
+        _parity(f"{wl} lane fp32_ir", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref,
+                y_frac=1.0, dT=1e-3, fc=0.95)

@@ -4,7 +4,9 @@ tests/cpu_emulation builds reactor_bdf.hip with the host compiler through a smal
 stand-in for the HIP runtime (one lane at a time), so the trip pipeline -- k_init,
 k_eval, k_setup, k_consume and the host loop -- is checked on every CPU test run.
 The device code follows the same algorithm as the oracle, so every cell must take the
-same number of right-hand-side evaluations and agree to rounding.
+same number of right-hand-side evaluations and agree to rounding.  The workspace starts
+filled with NaNs: device memory holds whatever was there before, and a word read before
+it is written (even multiplied by zero) must not reach a result.
 """
 import ctypes as C
 import glob
@@ -36,6 +38,9 @@ def emu():
         subprocess.run([CLANG, "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + EMU, "-o", LIB, SRC[0]], check=True)
     L = C.CDLL(LIB)
     L.emu_react.restype = C.c_int
+    # every workspace word starts as a NaN (and every list entry as -1), so a read of a
+    # word the integrator has not written poisons the cell instead of passing unseen
+    L.emu_set_fill(0xFF)
     return L
 
 
@@ -89,3 +94,29 @@ def test_predictor_plumbing_with_zero_network(emu):
     e_plain = np.abs(ref["T"] - tight["T"]).max()
     assert np.abs(T - tight["T"]).max() <= 5.0 * e_plain + 0.05
     assert np.all(FC > 0)
+
+
+@pytest.mark.parametrize("wl,mech,n", [("drm19_64", "drm19", 256), ("dodecane53_64_mix", "dodecane53", 64)])
+def test_emulated_fp32_factors_take_the_fp64_steps(emu, wl, mech, n):
+    """FP32 factors with FP64-residual refinement: on these cells the refinement brings
+    every solve to the FP64 solve's accuracy, so the step sequence (FC) is the FP64
+    form's and the states agree far inside the tolerance."""
+    d = cells.generate_workload(wl, ncell=n)
+    ref = O.Mech(mech).react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10, max_steps=20000)
+    rY, T, rE = d["rY"].copy(), d["T"].copy(), d["rE"].copy()
+    FC = np.zeros(n)
+    cnt = np.zeros(16, dtype=np.uint64)
+    trips = C.c_long(0)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    emu.emu_set_lu(1, 3)
+    try:
+        rc = emu.emu_react(mech.encode(), C.c_long(n), p(rY), p(T), p(rE), p(FC), C.c_double(d["dt"]),
+                           C.c_double(1e-6), C.c_double(1e-10), 4, 0, p(cnt), C.byref(trips))
+    finally:
+        emu.emu_set_lu(0, 3)
+    assert rc == 0 and cnt[8] == 0
+    assert cnt[10] > 0                          # refinement sweeps ran
+    assert np.mean(FC == ref["FC"]) >= 0.99
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    assert np.max(np.abs(rY - ref["rY"]) / tol) < 0.05
+    assert np.max(np.abs(T - ref["T"])) < 1e-6

@@ -182,3 +182,47 @@ def test_react_mask_skips_cells():
     np.testing.assert_array_equal(out["rY"][:, ::3], d["rY"][:, ::3])
     np.testing.assert_array_equal(out["FC"][::3], 0.0)
     assert np.all(out["FC"][1::3] > 0)
+
+
+@pytest.mark.parametrize("name", MECHS)
+def test_energy_row_follows_from_the_enthalpy(name):
+    """rho h = sum_k rhoY_k h_k(T) fixes the temperature row for any state and forcing:
+    differentiating in time, sum_k h_k f_k + rho cp f_T = F_h, with h_k and cp_k (erg/g)
+    from the NASA-7 thermo.  A row that drops the enthalpy the species forcing carries
+    (or any term of sum_k h_k d(rhoY_k)/dt) fails it."""
+    m = O.Mech(name)
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        rY, T = _random_state(m, rng)
+        y = np.r_[rY, T]
+        fY = rng.uniform(-1.0, 1.0, size=m.nspec) * 1e3 * rY
+        fH = rng.normal() * 1e9
+        f = m.rhs_fixedT(y, fY=fY, fH=fH)
+        cpR, hRT, _ = m.thermo(T)
+        h = hRT * m.Ru * T / m.W
+        cp = cpR * m.Ru / m.W
+        lhs = np.dot(h, f[:-1]) + np.dot(rY, cp) * f[-1]
+        scale = abs(fH) + np.sum(np.abs(h * f[:-1]))
+        assert abs(lhs - fH) <= 1e-12 * scale, (lhs, fH, scale)
+
+
+@pytest.mark.parametrize("wl,name", [("lidryer_16", "LiDryer"), ("drm19_64", "drm19"),
+                                     ("dodecane53_64_mix", "dodecane53")])
+def test_react_under_forcing_keeps_T_on_its_enthalpy(wl, name):
+    """After a forced advance the integrated temperature must be the one the advanced
+    (rho h, rhoY) define: T_out = HY2T(rhoh_out / rho_out, Y_out).  With species forcing
+    F_Y dt ~ 1e-4 rhoY and energy forcing, at rtol 1e-8, to 1e-6 K (a T row without the
+    forcing's enthalpy misses by about 0.2 K here)."""
+    m = O.Mech(name)
+    d = cells.generate_workload(wl, ncell=64)
+    dt = 1e-6
+    rng = np.random.default_rng(11)
+    fY = rng.uniform(-1.0, 1.0, size=d["rY"].shape) * 1e-4 * d["rY"] / dt
+    fH = rng.uniform(-1.0, 1.0, size=d["T"].shape) * 1e-4 * np.abs(d["rE"]) / dt
+    out = m.react(d["rY"], d["T"], d["rE"], dt, rYsrc=fY, rEsrc=fH, rtol=1e-8, atol=1e-14, max_steps=200000)
+    assert out["nfail"] == 0
+    rho = out["rY"].sum(axis=0)
+    np.testing.assert_allclose(out["rE"], d["rE"] + fH * dt, rtol=1e-14, atol=0)
+    for c in range(d["T"].shape[0]):
+        Tc, _ = m.hy2t(out["rE"][c] / rho[c], out["rY"][:, c] / rho[c], out["T"][c])
+        assert abs(out["T"][c] - Tc) <= 1e-6, (c, out["T"][c], Tc)
