@@ -43,6 +43,10 @@ struct Reactor {
   int lu_lane = 0;
   void* predictor = nullptr;    // attached network (mi_chem_attach_predictor), not owned
   int graph_trips = 0;          // 0: host-driven trips; K (even): graphs of K trips
+  // mi.schedule = persistent: waves (0: as many as the device holds resident), refill
+  // chunk, waves per SIMD the kernel is compiled for
+  int persist_waves = 0, persist_chunk = 8, persist_occ = 1;
+  int persist_cap[NMECH][2] = {};   // resident waves of the persistent kernel (0: not queried)
   hipStream_t capture_stream = nullptr;
   // mi.schedule = auto: the configuration chosen per box size, and the trial timings
   struct Tuned {
@@ -210,6 +214,59 @@ int autotune(Reactor* r, const ReactArgs& a0, hipStream_t stream, long max_trips
   return MI_OK;
 }
 
+// the return code of a finished call under mi.on_fail: one read of the failure counter
+// (the only synchronisation of a persistent call, and none under continue)
+int failure_code(Reactor* r, hipStream_t stream) {
+  if (r->on_fail_continue || r->null_reactor) return MI_OK;
+  unsigned long long nfail = 0;
+  HIPCHK(r, hipMemcpyAsync(&nfail, r->counters + CNT_NFAIL, sizeof(nfail), hipMemcpyDeviceToHost, stream));
+  HIPCHK(r, hipStreamSynchronize(stream));
+  if (nfail > 0)
+    return fail(r, MI_ECELLFAIL, std::to_string(nfail) + " cell(s) failed to integrate (FC < 0 marks them)");
+  return MI_OK;
+}
+
+// mi.schedule = persistent: the workspace holds the kernel's lane slots (whatever the box
+// size); the kernels are enqueued on `stream` with no host synchronisation
+int react_persist(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream) {
+  PersistPlan plan;
+  plan.occupancy = r->persist_occ;
+  plan.chunk = r->persist_chunk;
+  int& cap = r->persist_cap[r->mech][plan.occupancy - 1];
+  if (cap <= 0) cap = persist_capacity(r->mech, plan.occupancy);
+  if (cap <= 0) return fail(r, MI_EHIP, "cannot query the persistent kernel's occupancy");
+  // under stream capture (a caller building a graph) nothing may synchronise: no timing
+  // events and no failure read-back (failed cells still carry FC < 0)
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIPCHK(r, hipStreamIsCapturing(stream, &cs));
+  const bool capturing = cs != hipStreamCaptureStatusNone;
+  long waves = r->persist_waves > 0 ? r->persist_waves : cap;
+  waves = std::min(waves, (a.ncell + 63) / 64);
+  plan.waves = (int)std::max(1L, waves);
+  int rc = ensure_device_state(r, (long)plan.waves * 64);
+  if (rc) return rc;
+  a.ws = r->ws;
+  a.iws = r->iws;
+  a.mat = r->mat;
+  a.counts = r->counts;
+  a.counters = r->counters;
+  a.compact = 0;
+  a.use_pred = 0;
+  const long max_trips = r->max_trips > 0 ? r->max_trips : 3 * r->max_steps + 100;
+  r->last_schedule = SCHED_PERSIST;
+  r->last_graph = 0;
+  HIPCHK(r, launch_counters_reset(r->counters, stream));
+  if (!capturing) HIPCHK(r, hipEventRecord(r->ev0, stream));
+  HIPCHK(r, run_react_persist(r->mech, a, plan, max_trips, stream));
+  if (!capturing) HIPCHK(r, hipEventRecord(r->ev1, stream));
+  r->prof = KernelProf{};
+  r->last_stream = stream;
+  r->have_stats = true;
+  r->last_trips = -1;   // read from the device by mi_chem_get_stats
+  *time += *dt;
+  return capturing ? MI_OK : failure_code(r, stream);
+}
+
 int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream) {
   if (!r->inited) return fail(r, MI_ESTATE, "react called before init");
   if (!dt || !(*dt > 0.0)) return fail(r, MI_EINVAL, "dt must be positive");
@@ -234,6 +291,7 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
     *time += *dt;
     return MI_OK;
   }
+  if (r->schedule == SCHED_PERSIST && !r->null_reactor) return react_persist(r, a, dt, time, stream);
   int rc = ensure_device_state(r, r->null_reactor ? 0 : a.ncell);
   if (rc) return rc;
   a.ws = r->ws;
@@ -282,16 +340,7 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   r->last_stream = stream;
   r->have_stats = true;
   *time += *dt;
-  if (!r->on_fail_continue && !r->null_reactor) {
-    // the trips have already synchronised the stream once per trip: one more read of the
-    // failure counter decides the return code
-    unsigned long long nfail = 0;
-    HIPCHK(r, hipMemcpyAsync(&nfail, r->counters + CNT_NFAIL, sizeof(nfail), hipMemcpyDeviceToHost, stream));
-    HIPCHK(r, hipStreamSynchronize(stream));
-    if (nfail > 0)
-      return fail(r, MI_ECELLFAIL, std::to_string(nfail) + " cell(s) failed to integrate (FC < 0 marks them)");
-  }
-  return MI_OK;
+  return failure_code(r, stream);
 }
 
 ArrView flat_view(double* p, long ncell) {
@@ -391,7 +440,17 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
     if (v == "plain") r->schedule = SCHED_PLAIN;
     else if (v == "queue") r->schedule = SCHED_COMPACT;
     else if (v == "auto") r->schedule = SCHED_AUTO;
-    else return fail(r, MI_EINVAL, "mi.schedule must be plain, queue or auto");
+    else if (v == "persistent") r->schedule = SCHED_PERSIST;
+    else return fail(r, MI_EINVAL, "mi.schedule must be plain, queue, auto or persistent");
+  } else if (k == "mi.persist_waves") {
+    if (!parse_long(value, &l) || l < 0 || l > (1L << 20)) return fail(r, MI_EINVAL, "mi.persist_waves must be 0..2^20");
+    r->persist_waves = (int)l;
+  } else if (k == "mi.persist_chunk") {
+    if (!parse_long(value, &l) || l < 1 || l > 64) return fail(r, MI_EINVAL, "mi.persist_chunk must be 1..64");
+    r->persist_chunk = (int)l;
+  } else if (k == "mi.persist_occupancy") {
+    if (!parse_long(value, &l) || (l != 1 && l != 2)) return fail(r, MI_EINVAL, "mi.persist_occupancy must be 1 or 2");
+    r->persist_occ = (int)l;
   } else if (k == "mi.graph_trips") {
     if (!parse_long(value, &l) || l < 0 || l > 64 || (l & 1)) return fail(r, MI_EINVAL, "mi.graph_trips must be 0 or even up to 64");
     r->graph_trips = (int)l;
@@ -605,6 +664,11 @@ int mi_chem_get_stats(void* reactor, mi_chem_stats* out) {
   HIPCHK(r, hipEventElapsedTime(&ms, r->ev0, r->ev1));
   out->ms_kernel = ms;
   out->ntrips = r->last_trips;
+  if (r->last_schedule == SCHED_PERSIST) {   // the most iterations (trips) of a wave
+    unsigned int it = 0;
+    HIPCHK(r, hipMemcpy(&it, r->counts + 2, sizeof(it), hipMemcpyDeviceToHost));
+    out->ntrips = it;
+  }
   for (int kk = 0; kk < MI_NKERNEL_KINDS && kk < NKK; ++kk) {
     out->kernel_ms[kk] = r->prof.ms[kk];
     out->kernel_launches[kk] = r->prof.launches[kk];

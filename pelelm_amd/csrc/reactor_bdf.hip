@@ -1414,33 +1414,21 @@ struct Coop {
 // is partial pivoting with the interchanges kept as a permutation.  A launch handles
 // the pending cells whose factor precision is Real (FP32 under LU_FP32_IR unless the
 // cell has fallen back to FP64).
-template <class M, class Real>
-__device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __restrict__ list, const long n,
-                                             const long blk) {
-  constexpr int N1 = M::N1;
+// One cell's factorisation by its group of G lanes (lane r of the group holds row r):
+// `work` (uniform over the group) says whether the group factors at all, `mat` is the
+// cell's matrix region; every lane of the wave must call it (it synchronises the block).
+// Returns 0, or 1 + the first column without a nonzero pivot (uniform over the group).
+template <int N1, class Real>
+__device__ __forceinline__ int lu_coop_core(const CM& mat, const bool work, const double gamma) {
   constexpr int G = Coop<N1>::G, CPW = Coop<N1>::CPW;
-  using S = Slots<N1>;
   using A = LUAcc<N1, Real>;
   constexpr bool IS32 = sizeof(Real) == 4;
   __shared__ Real prow[CPW][N1];
   const int lane = threadIdx.x;
   const int half = lane / G, r = lane % G;
-  const long idx = blk * CPW + half;
-  long c = -1;
-  bool work = false;
-  double gamma = 0.0;
-  if (idx < n) {
-    c = list[idx];
-    const WS w = cell_ws<M>(a.ws, c);
-    const bool pend = LANE_FIELD(M, w, setup_pending) != 0;
-    const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
-    work = pend && (fp32_cell == IS32);
-    gamma = LANE_FIELD(M, w, gamma);
-  }
   const bool row = work && r < N1;
   Real m[N1];
   if (row) {
-    const CM mat = cell_mat<M>(a.mat, c);
 #pragma unroll
     for (int j = 0; j < N1; ++j) m[j] = (Real)((r == j ? 1.0 : 0.0) - gamma * mat[MatSlots<N1>::JS + r * N1 + j]);
   } else {
@@ -1512,14 +1500,44 @@ __device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __re
     }
   });
   if (row) {
-    const WS w = cell_ws<M>(a.ws, c);
-    const CM mat = cell_mat<M>(a.mat, c);
 #pragma unroll
     for (int j = 0; j < N1; ++j) A::at(mat, my_step * N1 + j) = m[j];
     mat.perm(MatSlots<N1>::PERM, my_step) = r;
-    if (r == 0) w[S::SING] = (double)sing;
   }
   __syncthreads();   // prow is reused by the next group of this block
+  return sing;
+}
+
+// Cooperative LU of M = I - gamma J in Real arithmetic: G lanes per cell (row r of M
+// in lane r), 64 / G cells per wave.  Column k's pivot is the largest |m_rk| over the rows
+// not yet pivoted (a G-lane max-reduction); its row is broadcast through LDS and every
+// other remaining row is eliminated in registers.  Row r ends as row `step` of the
+// factors, where `step` is the column at which it was chosen (PERM[step] = r), which
+// is partial pivoting with the interchanges kept as a permutation.  A launch handles
+// the pending cells whose factor precision is Real (FP32 under LU_FP32_IR unless the
+// cell has fallen back to FP64).
+template <class M, class Real>
+__device__ __forceinline__ void lu_coop_pair(const ReactArgs& a, const int* __restrict__ list, const long n,
+                                             const long blk) {
+  constexpr int N1 = M::N1;
+  constexpr int G = Coop<N1>::G, CPW = Coop<N1>::CPW;
+  using S = Slots<N1>;
+  constexpr bool IS32 = sizeof(Real) == 4;
+  const int half = threadIdx.x / G, r = threadIdx.x % G;
+  const long idx = blk * CPW + half;
+  long c = -1;
+  bool work = false;
+  double gamma = 0.0;
+  if (idx < n) {
+    c = list[idx];
+    const WS w = cell_ws<M>(a.ws, c);
+    const bool pend = LANE_FIELD(M, w, setup_pending) != 0;
+    const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
+    work = pend && (fp32_cell == IS32);
+    gamma = LANE_FIELD(M, w, gamma);
+  }
+  const int sing = lu_coop_core<N1, Real>(cell_mat<M>(a.mat, c < 0 ? 0 : c), work, gamma);
+  if (work && r == 0) cell_ws<M>(a.ws, c)[S::SING] = (double)sing;
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin: each XCD gets contiguous ranges of
@@ -1603,30 +1621,19 @@ __global__ void __launch_bounds__(64) k_solve_lane(const ReactArgs a, const int*
 
 // The same solves with G lanes per cell (row r of the factors in lane r), 64 / G cells
 // per wave: forward and back substitution broadcast each finished component with a
-// G-lane shuffle; a refinement residual gathers x the same way.  A launch handles the
-// cells whose factor precision is Real.  No barriers: the two halves of a wave may
-// take different numbers of refinement sweeps.
-template <class M, class Real>
-__device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* __restrict__ list, const long n,
-                                                const long blk) {
-  constexpr int N1 = M::N1;
-  constexpr int G = Coop<N1>::G, CPW = Coop<N1>::CPW;
-  using S = Slots<N1>;
+// G-lane shuffle; a refinement residual gathers x the same way.
+//
+// One cell's solve by its group: `work` (uniform over the group), the cell's matrix
+// region and, for FP32 factors, its error weights (ewt of row r in lane r) and gammap.
+template <int N1, class Real>
+__device__ __forceinline__ void solve_coop_core(const CM& mat, const bool work, const double ewt, const double gp,
+                                                const int ir_sweeps, const double ir_tol) {
+  constexpr int G = Coop<N1>::G;
   using MS = MatSlots<N1>;
   using A = LUAcc<N1, Real>;
   constexpr bool IS32 = sizeof(Real) == 4;
-  const int half = threadIdx.x / G, r = threadIdx.x % G;
-  const long idx = blk * CPW + half;
-  bool work = false;
-  long c = 0;
-  if (idx < n) {
-    c = list[idx];
-    const WS w = cell_ws<M>(a.ws, c);
-    const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
-    work = solve_due<M>(w) && (fp32_cell == IS32);
-  }
+  const int r = threadIdx.x % G;
   const bool row = work && r < N1;
-  const CM mat = cell_mat<M>(a.mat, c);
   Real lrow[N1];
   int pr = 0;
   if (row) {
@@ -1655,15 +1662,15 @@ __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* _
   double x = subst(row ? mat[MS::B + pr] : 0.0);
   if (IS32) {
     // refinement of the FP32 solve against the FP64 matrix (uniform per cell's lanes)
-    const WS w = cell_ws<M>(a.ws, c);
-    const double gp = work ? LANE_FIELD(M, w, gammap) : 0.0;
     const double bi = row ? mat[MS::B + r] : 0.0;
-    const double ewt = row ? w[S::EWT + r] : 0.0;
     int sweeps = 0;
-    bool ok = !work || (a.ir_sweeps == 0);
+    bool ok = !work || (ir_sweeps == 0);
+    // the loop runs while any group of the wave still refines (its cross-lane operations
+    // stay wave-uniform); a group that has finished computes a sweep it does not apply
 #pragma unroll 1
-    for (; sweeps < a.ir_sweeps && !ok;) {
-      ++sweeps;
+    for (;;) {
+      const bool more = !ok && sweeps < ir_sweeps;   // uniform over the group
+      if (__ballot(more) == 0ull) break;
       double res = bi - x;
 #pragma unroll
       for (int j = 0; j < N1; ++j) {
@@ -1671,12 +1678,15 @@ __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* _
         if (row) res += gp * mat[MS::JS + r * N1 + j] * xj;
       }
       const double d = subst(__shfl(res, pr, G));
-      x += d;
-      double q = d * ewt;
+      double q = row ? d * ewt : 0.0;
       q *= q;
 #pragma unroll
       for (int off = G / 2; off > 0; off >>= 1) q += __shfl_xor(q, off, G);
-      ok = (sqrt(q / N1) <= a.ir_tol);
+      if (more) {
+        ++sweeps;
+        x += d;
+        ok = (sqrt(q / N1) <= ir_tol);   // false for a NaN norm
+      }
     }
     if (work && r == 0) {
       mat[MS::IRS] = (double)sweeps;
@@ -1684,6 +1694,33 @@ __device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* _
     }
   }
   if (row) mat[MS::X + r] = x;
+}
+
+// the solves of a list's cells whose factor precision is Real
+template <class M, class Real>
+__device__ __forceinline__ void solve_coop_pair(const ReactArgs& a, const int* __restrict__ list, const long n,
+                                                const long blk) {
+  constexpr int N1 = M::N1;
+  constexpr int G = Coop<N1>::G, CPW = Coop<N1>::CPW;
+  using S = Slots<N1>;
+  constexpr bool IS32 = sizeof(Real) == 4;
+  const int half = threadIdx.x / G, r = threadIdx.x % G;
+  const long idx = blk * CPW + half;
+  bool work = false;
+  long c = 0;
+  if (idx < n) {
+    c = list[idx];
+    const WS w = cell_ws<M>(a.ws, c);
+    const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
+    work = solve_due<M>(w) && (fp32_cell == IS32);
+  }
+  double ewt = 0.0, gp = 0.0;
+  if (IS32 && work) {
+    const WS w = cell_ws<M>(a.ws, c);
+    gp = LANE_FIELD(M, w, gammap);
+    if (r < N1) ewt = w[S::EWT + r];
+  }
+  solve_coop_core<N1, Real>(cell_mat<M>(a.mat, c), work, ewt, gp, a.ir_sweeps, a.ir_tol);
 }
 
 template <class M, class Real, bool STRIDE>
@@ -1857,6 +1894,330 @@ __global__ void __launch_bounds__(64) k_abandon(const ReactArgs a, const int* __
     wave_add(&a.counters[CNT_NFAIL], failed ? 1ull : 0ull);
     wave_add(&a.counters[CNT_CELLS], failed ? 1ull : 0ull);
   }
+}
+
+// ---------------------------------------------------------------------------
+// the persistent schedule
+//
+// One kernel per call.  Each wave owns 64 lane slots (workspace tile and matrix region
+// per slot, so the accesses of one slot by the wave stay whole lines however the cells
+// are spread); a lane keeps its cell's integrator state (Lane<M>) in private memory for
+// the cell's whole integration, and every iteration of the wave's loop is one trip of
+// every cell it holds: evaluation, set-up, cooperative LU and solves over the wave's
+// own cells, step logic.  Idle lanes claim the next cells of the queue with one atomic
+// per claim; the wave ends when the queue is drained and its lanes are idle.  Every
+// cross-lane operation below is reached by all 64 lanes (wave-uniform control flow).
+// ---------------------------------------------------------------------------
+
+// the cells with mask >= 0 into the queue (in cell order, or in `order`); masked cells
+// get FC = 0
+template <class M>
+__global__ void __launch_bounds__(64) k_queue(const ReactArgs a, const int* __restrict__ order, int* queue,
+                                              unsigned int* qcount) {
+  const long idx = (long)blockIdx.x * 64 + threadIdx.x;
+  bool active = false;
+  long c = -1;
+  if (idx < a.ncell) {
+    c = order ? order[idx] : idx;
+    int ci, cj, ck;
+    cell_ijk(a, c, ci, cj, ck);
+    const int mk = a.mask.p ? a.mask.p[cell_off(a.mask, ci, cj, ck)] : 1;
+    if (mk < 0) {
+      a.FC.p[cell_off(a.FC, ci, cj, ck)] = 0.0;
+      if (a.cell_status) a.cell_status[c] = 0;
+    } else {
+      active = true;
+    }
+  }
+  wave_append(active, (int)c, queue, qcount);
+}
+
+// The wave's cells that `sel` marks, as a list of lanes in LDS; returns their count
+// (wave-uniform).  Called by all 64 lanes.
+__device__ __forceinline__ int wave_lanes(const bool sel, int* lanes) {
+  const unsigned long long b = __ballot(sel);
+  const int lane = threadIdx.x;
+  if (sel) lanes[__popcll(b & ((1ull << lane) - 1ull))] = lane;
+  __syncthreads();
+  return __popcll(b);
+}
+
+template <class M, int OCC>
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(OCC) k_persist(const ReactArgs a, const int* __restrict__ queue,
+                                                                      const unsigned int* __restrict__ qcount,
+                                                                      unsigned int* head, unsigned int* iters_max,
+                                                                      const int chunk, const long max_trips) {
+  constexpr int N = M::NS, N1 = M::N1;
+  constexpr int G = Coop<N1>::G, CPW = Coop<N1>::CPW;
+  using S = Slots<N1>;
+  using MS = MatSlots<N1>;
+  __shared__ double stage[64 * N1];   // rows between lanes and matrix regions
+  __shared__ long sslot[64];
+  __shared__ int slanes[64];
+  __shared__ double sgam[64];         // gamma, then gammap, of each lane's cell
+  __shared__ int ssing[64];
+  const int lane = threadIdx.x;
+  const int half = lane / G, r = lane % G;
+  const long wave0 = (long)blockIdx.x * 64;
+  const long slot = wave0 + lane;
+  const WS w = cell_ws<M>(a.ws, slot);
+  const CM mat = cell_mat<M>(a.mat, slot);
+  // with a queue, its entries are the cells (mask >= 0 already); without one the cells
+  // are claimed in cell order and masked cells are settled as they are claimed
+  const unsigned int n = queue ? *qcount : (unsigned int)a.ncell;
+  Lane<M> st;
+  st.ws = w;
+  st.iws = cell_iws<M>(a.iws, slot);
+  st.mat = mat;
+  st.xs = &stage[lane * N1];
+  st.qmax = a.qmax;
+  st.lu_mode = a.lu_mode;
+  st.ir_sweeps = a.ir_sweeps;
+  st.use_pred = 0;
+  st.rtol = a.rtol;
+  st.mxstep = a.mxstep;
+  st.tout = a.dt;
+  st.phase = PH_NEED_CELL;
+  st.setup_pending = 0;
+  st.lu64 = 0;
+  st.pred_state = 0;
+  long cell = -1, trips = 0;
+  int ci = 0, cj = 0, ck = 0;
+  bool exhausted = false;
+  // statistics of the cells this lane finishes, added to the call's counters at the end
+  unsigned long long t_cells = 0, t_nst = 0, t_nfe = 0, t_nje = 0, t_nsetups = 0, t_nni = 0, t_ncfn = 0,
+                     t_netf = 0, t_fail = 0, t_maxst = 0, t_ir = 0, t_lufb = 0, t_hy2t = 0;
+  unsigned int iters = 0;
+  auto lane_done = [&](const bool ok) {
+    a.FC.p[cell_off(a.FC, ci, cj, ck)] = ok ? (double)st.nfe : -(double)st.nfe;
+    if (a.cell_status) a.cell_status[cell] = st.status;
+    ++t_cells;
+    t_nst += st.nst; t_nfe += st.nfe; t_nje += st.nje; t_nsetups += st.nsetups; t_nni += st.nni;
+    t_ncfn += st.ncfn; t_netf += st.netf; t_fail += ok ? 0 : 1;
+    t_maxst = t_maxst > (unsigned long long)st.nst ? t_maxst : (unsigned long long)st.nst;
+    t_ir += st.nir; t_lufb += st.nlufb;
+    cell = -1;
+  };
+#pragma unroll 1
+  for (;; ++iters) {
+    // -- claim cells for idle lanes -------------------------------------------------
+    {
+      const bool idle = cell < 0;
+      const unsigned long long bidle = __ballot(idle);
+      const int nidle = __popcll(bidle);
+      if (!exhausted && nidle > 0 && (nidle >= chunk || nidle == 64)) {
+        const int leader = __ffsll((long long)bidle) - 1;
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(head, (unsigned int)nidle);
+        base = __shfl(base, leader, 64);
+        if (base + (unsigned int)nidle >= n) exhausted = true;
+        if (idle) {
+          const unsigned int idx = base + (unsigned int)__popcll(bidle & ((1ull << lane) - 1ull));
+          if (idx < n) {
+            cell = queue ? queue[idx] : (long)idx;
+            cell_ijk(a, cell, ci, cj, ck);
+            if (!queue && a.mask.p && a.mask.p[cell_off(a.mask, ci, cj, ck)] < 0) {
+              a.FC.p[cell_off(a.FC, ci, cj, ck)] = 0.0;
+              if (a.cell_status) a.cell_status[cell] = 0;
+              cell = -1;
+            }
+          }
+          if (cell >= 0) {
+#pragma unroll
+            for (int i = 0; i < N1; ++i) w[S::ATOL + i] = a.atol[i];
+            double y0[N1];
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+              y0[k] = a.rY.p[cell_off(a.rY, ci, cj, ck) + k * a.rY.ns];
+              w[S::FY + k] = a.rYsrc.p ? a.rYsrc.p[cell_off(a.rYsrc, ci, cj, ck) + k * a.rYsrc.ns] : 0.0;
+            }
+            y0[N] = a.T.p[cell_off(a.T, ci, cj, ck)];
+            st.rE0 = a.rE.p[cell_off(a.rE, ci, cj, ck)];
+            st.fH = a.rEsrc.p ? a.rEsrc.p[cell_off(a.rEsrc, ci, cj, ck)] : 0.0;
+            st.init_cell(y0);
+            trips = 0;
+          }
+        }
+      }
+      if (__ballot(cell >= 0) == 0ull) break;   // queue drained, every lane idle
+    }
+    // -- trip cap: a cell that has not finished after max_trips trips fails -----------
+    if (cell >= 0 && trips >= max_trips) {
+      st.finish(ST_TOO_MUCH_WORK);
+      lane_done(false);
+    }
+    const bool act = cell >= 0;
+    // -- evaluation: y = zn0 + a1 zn1 + a2 acor, f(teval, y); the Newton right-hand side
+    //    gamma f - rl1 zn1 - acor goes to the matrix region through LDS ---------------
+    double y[N1], f[N1], Tsol = 0.0;
+    bool newton = false;
+    if (act) {
+      int it = 0;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) y[i] = w[S::ZN + i] + st.a1 * w[S::ZN + N1 + i] + st.a2 * w[S::ACOR + i];
+      rhs<M>(y, st.teval, w, st.rE0, st.fH, f, Tsol, &it);
+      t_hy2t += (unsigned long long)it;
+      if (st.phase == PH_NEWTON) {
+        newton = true;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) stage[lane * N1 + i] = st.gamma * f[i] - st.rl1 * w[S::ZN + N1 + i] - w[S::ACOR + i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < N1; ++i) w[S::F + i] = f[i];
+      }
+    }
+    sslot[lane] = newton ? slot : -1;
+    __syncthreads();
+    rows_to_cells<N1>(stage, sslot, a.mat, MS::STRIDE, MS::B);
+    __syncthreads();
+    // -- set-up: the Jacobian (when stale) into the tile, then row by row to the matrix
+    //    region ---------------------------------------------------------------------
+    bool fresh = false;
+    if (act && st.needs_setup() && !st.setup_pending) {
+      st.setup(y, f, Tsol);
+      fresh = st.jcur;
+    }
+    if (__ballot(fresh) != 0ull) {
+      sslot[lane] = fresh ? slot : -1;
+#pragma unroll 1
+      for (int rr = 0; rr < N1; ++rr) {
+        __syncthreads();
+        if (fresh) {
+#pragma unroll
+          for (int j = 0; j < N1; ++j) stage[lane * N1 + j] = w[S::JS + rr * N1 + j];
+        }
+        __syncthreads();
+        rows_to_cells<N1>(stage, sslot, a.mat, MS::STRIDE, MS::JS + rr * N1);
+      }
+      __syncthreads();
+    }
+    // -- factorisations of this wave's pending cells, G lanes per cell -----------------
+    const bool pend = act && st.setup_pending;
+    const bool lu32 = pend && a.lu_mode == LU_FP32_IR && !st.lu64;
+    sgam[lane] = st.gamma;
+    ssing[lane] = 0;
+    {
+      const int cnt = wave_lanes(lu32, slanes);
+#pragma unroll 1
+      for (int b = 0; b < cnt; b += CPW) {
+        const bool work = b + half < cnt;
+        const int owner = work ? slanes[b + half] : 0;
+        const int sg = lu_coop_core<N1, float>(cell_mat<M>(a.mat, wave0 + owner), work, sgam[owner]);
+        if (work && r == 0) ssing[owner] = sg;
+      }
+      __syncthreads();
+    }
+    {
+      const int cnt = wave_lanes(pend && !lu32, slanes);
+#pragma unroll 1
+      for (int b = 0; b < cnt; b += CPW) {
+        const bool work = b + half < cnt;
+        const int owner = work ? slanes[b + half] : 0;
+        const int sg = lu_coop_core<N1, double>(cell_mat<M>(a.mat, wave0 + owner), work, sgam[owner]);
+        if (work && r == 0) ssing[owner] = sg;
+      }
+      __syncthreads();
+    }
+    const int sing = ssing[lane];
+    // -- Newton solves of this wave's cells that iterate this trip ----------------------
+    const bool due = newton && !(pend && sing != 0);
+    const bool s32 = due && a.lu_mode == LU_FP32_IR && !st.lu64;
+    sgam[lane] = st.gammap;
+    {
+      const int cnt = wave_lanes(s32, slanes);
+#pragma unroll 1
+      for (int b = 0; b < cnt; b += CPW) {
+        const bool work = b + half < cnt;
+        const int owner = work ? slanes[b + half] : 0;
+        const double ewt = (work && r < N1) ? cell_ws<M>(a.ws, wave0 + owner)[S::EWT + r] : 0.0;
+        solve_coop_core<N1, float>(cell_mat<M>(a.mat, wave0 + owner), work, ewt, sgam[owner], a.ir_sweeps, a.ir_tol);
+      }
+      __syncthreads();
+    }
+    {
+      const int cnt = wave_lanes(due && !s32, slanes);
+#pragma unroll 1
+      for (int b = 0; b < cnt; b += CPW) {
+        const bool work = b + half < cnt;
+        const int owner = work ? slanes[b + half] : 0;
+        solve_coop_core<N1, double>(cell_mat<M>(a.mat, wave0 + owner), work, 0.0, 0.0, 0, 0.0);
+      }
+      __syncthreads();
+    }
+    // -- step logic: the solutions come back through LDS -------------------------------
+    sslot[lane] = newton ? slot : -1;
+    __syncthreads();
+    cells_to_rows<N1>(stage, sslot, a.mat, MS::STRIDE, MS::X);
+    __syncthreads();
+    if (act) {
+      ++st.nfe;
+      ++trips;
+      bool reached = false, proceed = true;
+      if (st.setup_pending) proceed = st.after_setup(sing);
+      if (proceed) {
+        switch (st.phase) {
+          case PH_INIT_F0:
+#pragma unroll
+            for (int i = 0; i < N1; ++i) st.zn(1, i) = w[S::F + i];
+            st.after_f0();
+            break;
+          case PH_HIN:
+#pragma unroll
+            for (int i = 0; i < N1; ++i) f[i] = w[S::F + i];
+            st.after_hin(f);
+            break;
+          case PH_NEWTON:
+            reached = st.iterate();
+            break;
+          case PH_RESTART_F:
+#pragma unroll
+            for (int i = 0; i < N1; ++i) f[i] = w[S::F + i];
+            st.restart_f(f);
+            break;
+          default:
+            break;
+        }
+      }
+      if (reached) {
+        // CV_NORMAL output: interpolate the Nordsieck array back to tout
+        const double sfac = (st.tout - st.tn) / st.h;
+        double yo[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) yo[i] = st.zn(st.q, i);
+        for (int j = st.q - 1; j >= 0; --j)
+#pragma unroll
+          for (int i = 0; i < N1; ++i) yo[i] = st.zn(j, i) + sfac * yo[i];
+        bool finite = true;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) finite = finite && isfinite(yo[i]);
+        st.finish(finite ? ST_OK : ST_NONFINITE);
+        if (finite) {
+#pragma unroll
+          for (int k = 0; k < N; ++k) a.rY.p[cell_off(a.rY, ci, cj, ck) + k * a.rY.ns] = yo[k];
+          a.T.p[cell_off(a.T, ci, cj, ck)] = yo[N];
+          a.rE.p[cell_off(a.rE, ci, cj, ck)] = st.rE0 + st.fH * st.tout;
+        }
+      }
+      if (st.phase == PH_NEED_CELL) lane_done(st.status == ST_OK);
+    }
+    __syncthreads();   // stage and sslot are rewritten by the next trip
+  }
+  if (a.counters) {
+    wave_add(&a.counters[CNT_CELLS], t_cells);
+    wave_add(&a.counters[CNT_NST], t_nst);
+    wave_add(&a.counters[CNT_NFE], t_nfe);
+    wave_add(&a.counters[CNT_NJE], t_nje);
+    wave_add(&a.counters[CNT_NSETUPS], t_nsetups);
+    wave_add(&a.counters[CNT_NNI], t_nni);
+    wave_add(&a.counters[CNT_NCFN], t_ncfn);
+    wave_add(&a.counters[CNT_NETF], t_netf);
+    wave_add(&a.counters[CNT_NFAIL], t_fail);
+    wave_max(&a.counters[CNT_MAXSTEPS], t_maxst);
+    wave_add(&a.counters[CNT_IR], t_ir);
+    wave_add(&a.counters[CNT_LUFALLBACK], t_lufb);
+    wave_add(&a.counters[CNT_HY2T], t_hy2t);
+  }
+  if (lane == 0) atomicMax(iters_max, iters);
 }
 
 // list bookkeeping (small kernels rather than memsets, so the trip can be captured)
@@ -2237,6 +2598,45 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
   return hipSuccess;
 }
 
+template <class M, int OCC>
+hipError_t launch_persist_occ(const ReactArgs& a, const int* queue, const PersistPlan& plan, long max_trips,
+                              hipStream_t s) {
+  hipLaunchKernelGGL((k_persist<M, OCC>), dim3((unsigned)plan.waves), dim3(64), 0, s, a, queue, &a.counts[0],
+                     &a.counts[1], &a.counts[2], plan.chunk, max_trips);
+  return hipGetLastError();
+}
+
+// counts[0]: queue length, [1]: queue head, [2]: most iterations of a wave
+template <class M>
+hipError_t run_react_persist_t(const ReactArgs& a, const PersistPlan& plan, long max_trips, hipStream_t s) {
+  if (plan.waves <= 0 || plan.chunk < 1 || plan.chunk > 64 || (plan.occupancy != 1 && plan.occupancy != 2) ||
+      max_trips < 1)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, a.counts);
+  PELE_TRY(hipGetLastError());
+  const int* queue = nullptr;
+  if (plan.order) {   // the queue: the ordered cells with mask >= 0
+    hipLaunchKernelGGL(k_queue<M>, dim3((unsigned)((a.ncell + 63) / 64)), dim3(64), 0, s, a, plan.order, a.list[0],
+                       &a.counts[0]);
+    PELE_TRY(hipGetLastError());
+    queue = a.list[0];
+  }
+  return plan.occupancy == 2 ? launch_persist_occ<M, 2>(a, queue, plan, max_trips, s)
+                             : launch_persist_occ<M, 1>(a, queue, plan, max_trips, s);
+}
+
+template <class M>
+int persist_capacity_t(int occupancy) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const hipError_t e = occupancy == 2
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist<M, 2>, 64, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist<M, 1>, 64, 0);
+  if (e != hipSuccess) return 0;
+  return ncu * per_cu;
+}
+
 // PeleLM::RhoH_to_Temp per cell (getTfromHY, Source/PeleLM_K.H:511-530): Y = rhoY / rho
 // and h = rhoH / rho x 1e4 (MKS to erg/g), T re-solved by HY2T from the incoming T
 template <class M>
@@ -2332,6 +2732,25 @@ hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_
     case MECH_DRM19: return run_react_t<mech_drm19>(a, stream, max_trips, trips_out, prof, pred, plan);
     case MECH_DODECANE53: return run_react_t<mech_dodecane53>(a, stream, max_trips, trips_out, prof, pred, plan);
     default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t run_react_persist(int mech, const ReactArgs& a, const PersistPlan& plan, long max_trips,
+                             hipStream_t stream) {
+  switch (mech) {
+    case MECH_LIDRYER: return run_react_persist_t<mech_LiDryer>(a, plan, max_trips, stream);
+    case MECH_DRM19: return run_react_persist_t<mech_drm19>(a, plan, max_trips, stream);
+    case MECH_DODECANE53: return run_react_persist_t<mech_dodecane53>(a, plan, max_trips, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int persist_capacity(int mech, int occupancy) {
+  switch (mech) {
+    case MECH_LIDRYER: return persist_capacity_t<mech_LiDryer>(occupancy);
+    case MECH_DRM19: return persist_capacity_t<mech_drm19>(occupancy);
+    case MECH_DODECANE53: return persist_capacity_t<mech_dodecane53>(occupancy);
+    default: return 0;
   }
 }
 

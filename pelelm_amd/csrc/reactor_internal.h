@@ -25,7 +25,8 @@ struct IntView {
 
 // SCHED_AUTO: the first call of each box size times the plain and compacted schedules,
 // each host-driven and graph-captured, on a copy of its cells and keeps the fastest
-enum Schedule : int { SCHED_PLAIN = 0, SCHED_COMPACT = 1, SCHED_AUTO = 2 };
+// SCHED_PERSIST: one persistent kernel per call (run_react_persist)
+enum Schedule : int { SCHED_PLAIN = 0, SCHED_COMPACT = 1, SCHED_AUTO = 2, SCHED_PERSIST = 3 };
 // LU_FP64: FP64 factors of I - gamma J; LU_FP32_IR: FP32 factors with FP64-residual
 // iterative refinement of every solve (a cell whose refinement fails falls back to FP64)
 enum LuMode : int { LU_FP64 = 0, LU_FP32_IR = 1 };
@@ -108,6 +109,28 @@ struct TripPlan {
 // pred: the learned predictor's device view, required when a.use_pred = 1
 hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out,
                      KernelProf* prof, const MlpDev* pred = nullptr, TripPlan plan = TripPlan{});
+
+// The persistent schedule: one kernel whose waves keep cells in their lanes.  Lane
+// slot s (wave s / 64, lane s % 64) owns workspace tile s and matrix region s, so the
+// workspace holds waves x 64 cells however large the box; idle lanes claim the next
+// cells (in `order` when given, through a queue of the cells with mask >= 0; else in cell
+// order, masked cells settled as they are claimed) from a device counter whenever
+// `chunk` lanes of a wave are idle, and every
+// cell is carried through all its trips inside the kernel.  Nothing is read back by the
+// host: the kernel ends when the queue is drained, each cell being bounded by max_trips.
+struct PersistPlan {
+  int waves = 0;          // workgroups of one wave (the slots are waves x 64)
+  int chunk = 1;          // idle lanes of a wave that trigger a claim (1..64)
+  int occupancy = 1;      // waves per SIMD the kernel is compiled for (1 or 2)
+  const int* order = nullptr;   // optional: cells in the order they are handed out
+};
+// the persistent kernels' maximum resident waves on this device for an occupancy
+int persist_capacity(int mech, int occupancy);
+// a.ws / a.mat hold plan.waves x 64 slots; with plan.order, a.list[0] holds ncell
+// entries (the queue); counts[0..3] are the queue length, its head, the largest
+// iteration count of a wave and a spare
+hipError_t run_react_persist(int mech, const ReactArgs& a, const PersistPlan& plan, long max_trips,
+                             hipStream_t stream);
 hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t stream);
 hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T, const int* mask, double* rYdot,
                         long ncell, hipStream_t stream);

@@ -1,0 +1,120 @@
+"""The reactor's cooperative kernels run on the CPU with wave semantics, against the
+oracle.
+
+tests/cpu_emulation/simt/hip/hip_runtime.h runs a workgroup's threads as fibers that
+meet at every cross-lane operation (ballot, shuffle, readlane, barrier), so the kernels
+that need a whole wave -- the LU and solves with 32 or 64 lanes per cell, and the
+persistent schedule whose waves claim cells from a device queue and keep them through
+all their trips -- run here as written for the GPU.  A lane that reaches a different
+cross-lane operation than the rest of its wave stops the run.  The workspace starts
+as NaNs.  The step sequence must be the oracle's (same RHS count per cell).
+"""
+import ctypes as C
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pelelm_amd import cells
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU = os.path.join(HERE, "cpu_emulation")
+LIB = os.path.join(EMU, "_build", "libsimt.so")
+CSRC = os.path.join(os.path.dirname(HERE), "pelelm_amd", "csrc")
+SRC = [os.path.join(EMU, "emu_simt.cpp"), os.path.join(EMU, "simt", "hip", "hip_runtime.h"),
+       os.path.join(CSRC, "reactor_bdf.hip"), os.path.join(CSRC, "reactor_internal.h"),
+       os.path.join(CSRC, "mlp_internal.h")] + sorted(glob.glob(os.path.join(CSRC, "mech", "*.inc")))
+CLANG = "/opt/rocm/llvm/bin/clang++"
+
+
+@pytest.fixture(scope="module")
+def simt():
+    stale = not os.path.exists(LIB) or any(os.path.getmtime(s) > os.path.getmtime(LIB) for s in SRC)
+    if stale:
+        os.makedirs(os.path.dirname(LIB), exist_ok=True)
+        subprocess.run([CLANG, "-O2", "-std=c++17", "-pthread", "-fPIC", "-shared", "-I" + os.path.join(EMU, "simt"),
+                        "-o", LIB, SRC[0]], check=True)
+    L = C.CDLL(LIB)
+    L.simt_react_trips.restype = C.c_int
+    L.simt_react_persist.restype = C.c_int
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(label, d, ref, rY, T, FC, cnt, n, lu):
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    ey = float((np.abs(rY - ref["rY"]) / tol).max())
+    et = float(np.abs(T - ref["T"]).max())
+    print(f"{label}: max |dY|/tol {ey:.3g}, max |dT| {et:.3g} K")
+    assert cnt[0] == n and cnt[8] == 0
+    np.testing.assert_array_equal(FC, ref["FC"])
+    assert ey < (1e-3 if lu == 0 else 1e-2) and et < 1e-8
+    assert cnt[2] == FC.sum()
+
+
+@pytest.mark.parametrize("wl,mech,n", [("drm19_64", "drm19", 128), ("dodecane53_64_mix", "dodecane53", 64)])
+@pytest.mark.parametrize("compact", [0, 1])
+def test_cooperative_trip_kernels_match_oracle(simt, wl, mech, n, compact):
+    d = cells.generate_workload(wl, ncell=n)
+    ref = O.Mech(mech).react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10, max_steps=20000)
+    rY, T, rE = d["rY"].copy(), d["T"].copy(), d["rE"].copy()
+    FC = np.zeros(n)
+    cnt = np.zeros(16, dtype=np.uint64)
+    trips = C.c_long(0)
+    rc = simt.simt_react_trips(mech.encode(), C.c_long(n), _p(rY), _p(T), _p(rE), _p(FC), C.c_double(d["dt"]),
+                               C.c_double(1e-6), C.c_double(1e-10), compact, 0, _p(cnt), C.byref(trips))
+    assert rc == 0
+    _check(f"{wl} trips compact={compact}", d, ref, rY, T, FC, cnt, n, 0)
+
+
+@pytest.mark.parametrize("wl,mech,n,waves,chunk", [("drm19_128_mix", "drm19", 256, 2, 1),
+                                                   ("drm19_128_mix", "drm19", 256, 2, 16),
+                                                   ("lidryer_16", "LiDryer", 300, 1, 64),
+                                                   ("dodecane53_64_mix", "dodecane53", 72, 1, 4)])
+@pytest.mark.parametrize("lu", [0, 1])
+def test_persistent_kernel_matches_oracle(simt, wl, mech, n, waves, chunk, lu):
+    """Fewer lane slots than cells, so lanes refill as their cells finish; the FP32
+    factors with refinement take the FP64 steps here too."""
+    d = cells.generate_workload(wl, ncell=n)
+    ref = O.Mech(mech).react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10, max_steps=20000)
+    rY, T, rE = d["rY"].copy(), d["T"].copy(), d["rE"].copy()
+    FC = np.zeros(n)
+    cnt = np.zeros(16, dtype=np.uint64)
+    iters = C.c_long(0)
+    rc = simt.simt_react_persist(mech.encode(), C.c_long(n), _p(rY), _p(T), _p(rE), _p(FC), C.c_double(d["dt"]),
+                                 C.c_double(1e-6), C.c_double(1e-10), waves, chunk, lu, C.c_long(60100), _p(cnt),
+                                 C.byref(iters))
+    assert rc == 0
+    _check(f"{wl} persistent waves={waves} chunk={chunk} lu={lu}", d, ref, rY, T, FC, cnt, n, lu)
+    # every iteration is one trip of each cell a wave holds: the longest wave needs at
+    # least the total work of its slots, and refills keep it near that
+    per_slot = ref["FC"].sum() / (waves * 64)
+    assert iters.value >= min(per_slot, ref["FC"].max())
+    if lu == 1:
+        assert cnt[10] > 0
+
+
+def test_persistent_trip_cap_fails_the_unfinished_cells(simt):
+    """max_trips = 5: cells still integrating after five trips are marked failed (FC =
+    -(RHS count)) and keep their input state; the kernel still drains the queue."""
+    n = 200
+    d = cells.generate_workload("drm19_128_mix", ncell=n)
+    rY, T, rE = d["rY"].copy(), d["T"].copy(), d["rE"].copy()
+    FC = np.zeros(n)
+    cnt = np.zeros(16, dtype=np.uint64)
+    iters = C.c_long(0)
+    rc = simt.simt_react_persist(b"drm19", C.c_long(n), _p(rY), _p(T), _p(rE), _p(FC), C.c_double(d["dt"]),
+                                 C.c_double(1e-6), C.c_double(1e-10), 1, 1, 0, C.c_long(5), _p(cnt),
+                                 C.byref(iters))
+    assert rc == 0
+    failed = FC < 0
+    assert cnt[0] == n and cnt[8] == failed.sum() > 0
+    assert np.all(np.abs(FC) <= 5)
+    np.testing.assert_array_equal(rY[:, failed], d["rY"][:, failed])
+    np.testing.assert_array_equal(T[failed], d["T"][failed])
