@@ -131,12 +131,23 @@ int mi_chem_react_flat(void* reactor, double* rY, const double* rYsrc, double* T
                        const double* rEsrc, double* FC, const int* mask, int ncells, double* dt, double* time,
                        void* hip_stream);
 
-/* Net production rates rho*wdot (CGS, g/cm^3/s) of every cell from (rhoY, rhoH, T):
-   T is first re-solved from h = rhoH/rho (Source/PeleLM_K.H:57-98 reactionRateRhoY, and
-   the T-from-(h,Y) update of Source/PeleLM_K.H:511-530).  SoA flat layout; T is updated
-   in place with the re-solved temperature; mask < 0 cells get zero rates. */
-int mi_chem_reaction_rates_flat(void* reactor, const double* rY, const double* rH, double* T,
-                                const int* mask, double* rYdot, int ncells, void* hip_stream);
+/* Net production rates rho*wdot (CGS, g/cm^3/s) of flat cells from (rhoY, rhoH, T) --
+   reactionRateRhoY (Source/PeleLM_K.H:57-98) in CGS units: T is re-solved from
+   h = rhoH/rho by HY2T starting from T, which is read only; the re-solved temperatures
+   are written to T_out when it is not NULL.  SoA flat layout (rY[k*ncells + c]);
+   mask < 0 cells (nullable mask) get zero rates. */
+int mi_chem_reaction_rates_flat(void* reactor, const double* rY, const double* rH, const double* T,
+                                const int* mask, double* rYdot, double* T_out, int ncells, void* hip_stream);
+
+/* PeleLM::compute_instantaneous_reaction_rates on one box (Source/PeleLM.cpp:2013-2073;
+   the per-cell reactionRateRhoY of Source/PeleLM_K.H:57-98, called at
+   Source/PeleLM.cpp:2046-2055).  MKS in and out, as PeleLM's state holds them: rho = the
+   sum of rhoY (nspec components), Y = rhoY / rho, h = rhoH / rho x 1e4, T re-solved by
+   HY2T from the T it holds (T is read only), and rhoYdot_k = W_k wdot_k(rho x 1e-3, T, Y)
+   x 1e3 (kg/m^3/s).  Cells with mask < 0 (EB-covered; mask.p NULL = all active) get zero
+   rates.  Asynchronous on hip_stream. */
+int mi_chem_reaction_rates_box(void* reactor, const int lo[3], const int hi[3], mi_real4 rhoY, mi_real4 rhoH,
+                               mi_real4 T, mi_int4 mask, mi_real4 rhoYdot, void* hip_stream);
 
 /* PeleLM::advance_chemistry on one box -- its per-box part, Source/PeleLM.cpp:5626-5844
    without the parallel copies: rhoY x 1e-3 and rhoH x 10 (MKS -> CGS, with the forcing

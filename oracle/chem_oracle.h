@@ -92,6 +92,12 @@ int orc_react(const orc_mech* m, int ncell, double* rY, const double* rYsrc, dou
               const double* rEsrc, double* FC, const int* mask, double dt, const orc_opts* opts,
               int nthreads, orc_cellstats* stats);
 
+/* The same advance (scalar tolerances, no mask) with the three-stage Radau IIA method of
+   order 5 (radau5.cpp), an integrator independent of the BDF.  nsteps (nullable) gets
+   each cell's accepted steps (-1 on failure).  Returns the number of failed cells. */
+int orc_radau_react(const orc_mech* m, int ncell, double* rY, const double* rYsrc, double* T, double* rE,
+                    const double* rEsrc, double dt, double rtol, double atol, int nthreads, long* nsteps);
+
 /* Integrate the linear test system y' = A y (n x n, row-major) with the same BDF
    machinery (pins the integrator against exp(A t)).  Returns status. */
 int orc_bdf_linear(int n, const double* A, double* y, double tout, const orc_opts* opts, orc_cellstats* st);

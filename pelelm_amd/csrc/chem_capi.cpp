@@ -364,7 +364,19 @@ ArrView box_view(const mi_real4& x, const int lo[3]) {
   return v;
 }
 
-bool box_covers(const mi_real4& x, const int lo[3], const int hi[3], int ncomp) {
+IntView box_iview(const mi_int4& x, const int lo[3]) {
+  IntView v{};
+  v.p = x.p;
+  v.js = x.jstride;
+  v.ks = x.kstride;
+  v.ox = lo[0] - x.begin[0];
+  v.oy = lo[1] - x.begin[1];
+  v.oz = lo[2] - x.begin[2];
+  return v;
+}
+
+template <class V>
+bool box_covers(const V& x, const int lo[3], const int hi[3], int ncomp) {
   if (!x.p) return true;
   for (int d = 0; d < 3; ++d)
     if (lo[d] < x.begin[d] || hi[d] >= x.end[d]) return false;
@@ -524,7 +536,8 @@ int mi_chem_react_box(void* reactor, const int lo[3], const int hi[3], mi_real4 
     if (hi[d] < lo[d]) return fail(r, MI_EINVAL, "empty box");
   const int N = mech_nspec(r->mech);
   if (!box_covers(rY, lo, hi, N) || !box_covers(rYsrc, lo, hi, N) || !box_covers(T, lo, hi, 1) ||
-      !box_covers(rE, lo, hi, 1) || !box_covers(rEsrc, lo, hi, 1) || !box_covers(FC, lo, hi, 1))
+      !box_covers(rE, lo, hi, 1) || !box_covers(rEsrc, lo, hi, 1) || !box_covers(FC, lo, hi, 1) ||
+      !box_covers(mask, lo, hi, 1))
     return fail(r, MI_EINVAL, "an array view does not cover the box or has too few components");
   ReactArgs a{};
   a.rY = box_view(rY, lo);
@@ -533,25 +546,59 @@ int mi_chem_react_box(void* reactor, const int lo[3], const int hi[3], mi_real4 
   a.rE = box_view(rE, lo);
   a.rEsrc = box_view(rEsrc, lo);
   a.FC = box_view(FC, lo);
-  a.mask.p = mask.p;
-  a.mask.js = mask.jstride;
-  a.mask.ks = mask.kstride;
-  a.mask.ox = lo[0] - mask.begin[0];
-  a.mask.oy = lo[1] - mask.begin[1];
-  a.mask.oz = lo[2] - mask.begin[2];
+  a.mask = box_iview(mask, lo);
   a.nx = hi[0] - lo[0] + 1;
   a.ny = hi[1] - lo[1] + 1;
   a.ncell = (long)a.nx * a.ny * (hi[2] - lo[2] + 1);
   return react_common(r, a, dt, time, static_cast<hipStream_t>(hip_stream));
 }
 
-int mi_chem_reaction_rates_flat(void* reactor, const double* rY, const double* rH, double* T, const int* mask,
-                                double* rYdot, int ncells, void* hip_stream) {
+int mi_chem_reaction_rates_flat(void* reactor, const double* rY, const double* rH, const double* T,
+                                const int* mask, double* rYdot, double* T_out, int ncells, void* hip_stream) {
   auto* r = static_cast<Reactor*>(reactor);
   if (!r) return MI_EINVAL;
   if (ncells < 0 || !rY || !rH || !T || !rYdot) return fail(r, MI_EINVAL, "bad arguments");
   if (ncells == 0) return MI_OK;
-  HIPCHK(r, launch_rates(r->mech, rY, rH, T, mask, rYdot, ncells, static_cast<hipStream_t>(hip_stream)));
+  RatesArgs ra;
+  ra.rhoY = flat_view(const_cast<double*>(rY), ncells);
+  ra.rhoH = flat_view(const_cast<double*>(rH), ncells);
+  ra.T = flat_view(const_cast<double*>(T), ncells);
+  ra.rdot = flat_view(rYdot, ncells);
+  ra.mask.p = mask;
+  ra.Tout = T_out;
+  ra.nx = ncells;
+  ra.ncell = ncells;
+  HIPCHK(r, launch_rates(r->mech, ra, static_cast<hipStream_t>(hip_stream)));
+  return MI_OK;
+}
+
+int mi_chem_reaction_rates_box(void* reactor, const int lo[3], const int hi[3], mi_real4 rhoY, mi_real4 rhoH,
+                               mi_real4 T, mi_int4 mask, mi_real4 rhoYdot, void* hip_stream) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (!lo || !hi) return fail(r, MI_EINVAL, "lo and hi are required");
+  for (int d = 0; d < 3; ++d)
+    if (hi[d] < lo[d]) return fail(r, MI_EINVAL, "empty box");
+  const int N = mech_nspec(r->mech);
+  if (!rhoY.p || !rhoH.p || !T.p || !rhoYdot.p) return fail(r, MI_EINVAL, "rhoY, rhoH, T and rhoYdot are required");
+  if (!box_covers(rhoY, lo, hi, N) || !box_covers(rhoH, lo, hi, 1) || !box_covers(T, lo, hi, 1) ||
+      !box_covers(rhoYdot, lo, hi, N) || !box_covers(mask, lo, hi, 1))
+    return fail(r, MI_EINVAL, "an array view does not cover the box or has too few components");
+  RatesArgs ra;
+  ra.rhoY = box_view(rhoY, lo);
+  ra.rhoH = box_view(rhoH, lo);
+  ra.T = box_view(T, lo);
+  ra.rdot = box_view(rhoYdot, lo);
+  ra.mask = box_iview(mask, lo);
+  ra.nx = hi[0] - lo[0] + 1;
+  ra.ny = hi[1] - lo[1] + 1;
+  ra.ncell = (long)ra.nx * ra.ny * (hi[2] - lo[2] + 1);
+  // PeleLM's MKS state: h = rhoH / rho x 1e4 (J/kg -> erg/g), rho x 1e-3 (kg/m^3 -> g/cm^3),
+  // rates x 1e3 (g/cm^3/s -> kg/m^3/s), Source/PeleLM_K.H:87-97
+  ra.hscale = 1.0e4;
+  ra.rscale = 1.0e-3;
+  ra.oscale = 1.0e3;
+  HIPCHK(r, launch_rates(r->mech, ra, static_cast<hipStream_t>(hip_stream)));
   return MI_OK;
 }
 

@@ -2242,34 +2242,46 @@ __global__ void k_plain_counts(unsigned int* counts, unsigned int n) {
 }
 
 // net production rates rho*wdot (g/cm^3/s) with T re-solved from h
+// reactionRateRhoY per cell (Source/PeleLM_K.H:57-98): rho = sum_k rhoY_k, Y = rhoY / rho,
+// h = rhoH / rho x hscale, T = HY2T(h, Y) started from the T the cell holds (T itself is
+// read only; the re-solved value goes to Tout when given), and
+// rhoYdot_k = W_k wdot_k(rho x rscale, T, Y) x oscale.  The box form passes PeleLM's MKS
+// factors (hscale 1e4, rscale 1e-3, oscale 1e3), the flat CGS form 1, 1, 1.  Cells with
+// mask < 0 (EB-covered) get zero rates.
 template <class M>
-__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) rates_kernel(const double* __restrict__ rY, const double* __restrict__ rH,
-                                                   double* __restrict__ Tio, const int* __restrict__ mask,
-                                                   double* __restrict__ rYdot, long ncell) {
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) rates_kernel(const ArrView rhoY, const ArrView rhoH,
+                                                                        const ArrView T, const IntView mask,
+                                                                        const ArrView rdot, double* __restrict__ Tout,
+                                                                        const int nx, const int ny, const long ncell,
+                                                                        const double hscale, const double rscale,
+                                                                        const double oscale) {
   constexpr int N = M::NS;
   const long c = (long)blockIdx.x * 64 + threadIdx.x;
   if (c >= ncell) return;
-  if (mask && mask[c] < 0) {
+  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long)nx * ny));
+  const long od = cell_off(rdot, i, j, k);
+  if (mask.p && mask.p[cell_off(mask, i, j, k)] < 0) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) rYdot[k * ncell + c] = 0.0;
+    for (int n = 0; n < N; ++n) rdot.p[od + n * rdot.ns] = 0.0;
     return;
   }
-  double y[N], Y[N], rho = 0.0;
+  const long oy = cell_off(rhoY, i, j, k);
+  double Y[N], rho = 0.0;
 #pragma unroll
-  for (int k = 0; k < N; ++k) { y[k] = rY[k * ncell + c]; rho += y[k]; }
+  for (int n = 0; n < N; ++n) { Y[n] = rhoY.p[oy + n * rhoY.ns]; rho += Y[n]; }
   const double irho = 1.0 / rho;
 #pragma unroll
-  for (int k = 0; k < N; ++k) Y[k] = y[k] * irho;
-  const double T = hy2t<M>(rH[c] * irho, Y, Tio[c]);
-  Tio[c] = T;
-  const double logT = log(T), invT = 1.0 / T;
+  for (int n = 0; n < N; ++n) Y[n] *= irho;
+  const double Tc = hy2t<M>(rhoH.p[cell_off(rhoH, i, j, k)] * irho * hscale, Y, T.p[cell_off(T, i, j, k)]);
+  if (Tout) Tout[c] = Tc;
+  const double rho_c = rho * rscale, logT = log(Tc), invT = 1.0 / Tc;
   double C[N], g[N], wd[N];
 #pragma unroll
-  for (int k = 0; k < N; ++k) C[k] = y[k] * M::imw(k);
-  M::gibbs(T, logT, g);
-  M::wdot(C, T, logT, invT, g, wd);
+  for (int n = 0; n < N; ++n) C[n] = rho_c * Y[n] * M::imw(n);
+  M::gibbs(Tc, logT, g);
+  M::wdot(C, Tc, logT, invT, g, wd);
 #pragma unroll
-  for (int k = 0; k < N; ++k) rYdot[k * ncell + c] = wd[k] * M::mw(k);
+  for (int n = 0; n < N; ++n) rdot.p[od + n * rdot.ns] = wd[n] * M::mw(n) * oscale;
 }
 
 __global__ void counters_reset_kernel(unsigned long long* counters) {
@@ -2428,7 +2440,9 @@ hipError_t enqueue_fixed_trip(const ReactArgs& a, int cur, hipStream_t s) {
   hipLaunchKernelGGL(k_setup<M>, dim3(nb), dim3(64), 0, s, a, a.slist[cur], &a.counts[2 + cur]);
   PELE_TRY(hipGetLastError());
   PELE_TRY(launch_lu<M>(a, a.slist[cur], &a.counts[2 + cur], n, s, false));
-  PELE_TRY(launch_solve<M>(a, 16384, s));
+  // the cooperative solve strides over its list from a fixed grid; the one-lane solve
+  // takes one cell per lane, so its grid covers every cell the list can hold
+  PELE_TRY(launch_solve<M>(a, a.lu_lane ? n : 16384, s));
   if (a.compact)
     hipLaunchKernelGGL(k_consume<M>, dim3(nb), dim3(64), 0, s, a, act, act_n, a.list[nxt], &a.counts[nxt],
                        a.slist[nxt], &a.counts[2 + nxt]);
@@ -2665,10 +2679,9 @@ hipError_t launch_rhoh_to_temp_t(const ArrView& rho, const ArrView& rhoY, const 
 }
 
 template <class M>
-hipError_t launch_rates_t(const double* rY, const double* rH, double* T, const int* mask, double* rYdot, long ncell,
-                          hipStream_t stream) {
-  const long nb = (ncell + 63) / 64;
-  hipLaunchKernelGGL(rates_kernel<M>, dim3((unsigned)nb), dim3(64), 0, stream, rY, rH, T, mask, rYdot, ncell);
+hipError_t launch_rates_t(const RatesArgs& ra, hipStream_t stream) {
+  hipLaunchKernelGGL(rates_kernel<M>, dim3((unsigned)((ra.ncell + 63) / 64)), dim3(64), 0, stream, ra.rhoY, ra.rhoH,
+                     ra.T, ra.mask, ra.rdot, ra.Tout, ra.nx, ra.ny, ra.ncell, ra.hscale, ra.rscale, ra.oscale);
   return hipGetLastError();
 }
 
@@ -2754,12 +2767,11 @@ int persist_capacity(int mech, int occupancy) {
   }
 }
 
-hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T, const int* mask, double* rYdot,
-                        long ncell, hipStream_t stream) {
+hipError_t launch_rates(int mech, const RatesArgs& ra, hipStream_t stream) {
   switch (mech) {
-    case MECH_LIDRYER: return launch_rates_t<mech_LiDryer>(rY, rH, T, mask, rYdot, ncell, stream);
-    case MECH_DRM19: return launch_rates_t<mech_drm19>(rY, rH, T, mask, rYdot, ncell, stream);
-    case MECH_DODECANE53: return launch_rates_t<mech_dodecane53>(rY, rH, T, mask, rYdot, ncell, stream);
+    case MECH_LIDRYER: return launch_rates_t<mech_LiDryer>(ra, stream);
+    case MECH_DRM19: return launch_rates_t<mech_drm19>(ra, stream);
+    case MECH_DODECANE53: return launch_rates_t<mech_dodecane53>(ra, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -132,8 +132,16 @@ int persist_capacity(int mech, int occupancy);
 hipError_t run_react_persist(int mech, const ReactArgs& a, const PersistPlan& plan, long max_trips,
                              hipStream_t stream);
 hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t stream);
-hipError_t launch_rates(int mech, const double* rY, const double* rH, double* T, const int* mask, double* rYdot,
-                        long ncell, hipStream_t stream);
+// reactionRateRhoY over a box or a flat array of cells (rates_kernel)
+struct RatesArgs {
+  ArrView rhoY, rhoH, T, rdot;
+  IntView mask;            // mask.p may be null (every cell active)
+  double* Tout = nullptr;  // optional: the re-solved T of each cell, flat
+  int nx = 1, ny = 1;
+  long ncell = 0;
+  double hscale = 1.0, rscale = 1.0, oscale = 1.0;
+};
+hipError_t launch_rates(int mech, const RatesArgs& ra, hipStream_t stream);
 hipError_t launch_null(int nspec, const ReactArgs& a, hipStream_t stream);
 // copy a call's rY [nspec][ncell], T and rE [ncell] into flat device arrays
 hipError_t launch_rhoh_to_temp(int mech, const ArrView& rho, const ArrView& rhoY, const ArrView& rhoH,

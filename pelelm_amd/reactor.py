@@ -46,7 +46,8 @@ class MiStats(C.Structure):
 
 # Every exported entry point of include/pelelm_chem.h.
 EXPORTS = ["mi_chem_create", "mi_chem_init", "mi_chem_set", "mi_chem_set_typ_vals", "mi_chem_react_box",
-           "mi_chem_react_flat", "mi_chem_reaction_rates_flat", "mi_chem_advance_box", "mi_chem_rhoh_to_temp_box",
+           "mi_chem_react_flat", "mi_chem_reaction_rates_flat", "mi_chem_reaction_rates_box", "mi_chem_advance_box",
+           "mi_chem_rhoh_to_temp_box",
            "mi_chem_get_stats", "mi_chem_nspec", "mi_chem_last_error", "mi_chem_close"]
 
 _lib = None
@@ -68,7 +69,9 @@ def lib():
                                         MiReal4, MiInt4, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
         L.mi_chem_react_flat.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), vp]
-        L.mi_chem_reaction_rates_flat.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, vp]
+        L.mi_chem_reaction_rates_flat.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
+        L.mi_chem_reaction_rates_box.argtypes = [vp, C.c_int * 3, C.c_int * 3, MiReal4, MiReal4, MiReal4, MiInt4,
+                                                 MiReal4, vp]
         L.mi_chem_advance_box.argtypes = [vp, C.c_int * 3, C.c_int * 3, MiReal4, MiReal4, MiReal4, MiReal4, MiReal4,
                                           MiInt4, C.c_double, vp]
         L.mi_chem_rhoh_to_temp_box.argtypes = [vp, C.c_int * 3, C.c_int * 3, MiReal4, MiReal4, MiReal4, MiReal4, vp]
@@ -195,10 +198,21 @@ class ReactorBase:
         self._check(lib().mi_chem_rhoh_to_temp_box(self._h, (C.c_int * 3)(*lo), (C.c_int * 3)(*hi), rho, rhoY, rhoH,
                                                    T, _stream(stream)))
 
-    def reaction_rates(self, rY, rH, T, mask, rYdot, stream=None):
+    def reaction_rates(self, rY, rH, T, mask, rYdot, T_out=None, stream=None):
+        """Net production rates rho*wdot of flat CGS cells (reactionRateRhoY,
+        Source/PeleLM_K.H:57-98, in CGS): T is the HY2T guess and is not written; the
+        re-solved temperatures go to T_out when given."""
         ncell = T.shape[0]
         self._check(lib().mi_chem_reaction_rates_flat(self._h, _ptr(rY), _ptr(rH), _ptr(T), _ptr(mask),
-                                                      _ptr(rYdot), ncell, _stream(stream)))
+                                                      _ptr(rYdot), _ptr(T_out), ncell, _stream(stream)))
+
+    def reaction_rates_box(self, lo, hi, rhoY: MiReal4, rhoH: MiReal4, T: MiReal4, mask: MiInt4, rhoYdot: MiReal4,
+                           stream=None):
+        """PeleLM::compute_instantaneous_reaction_rates on one box (reactionRateRhoY,
+        Source/PeleLM_K.H:57-98, called at Source/PeleLM.cpp:2046-2055): MKS rhoY, rhoH
+        and T (read only) in, MKS rhoYdot out, zero where mask < 0."""
+        self._check(lib().mi_chem_reaction_rates_box(self._h, (C.c_int * 3)(*lo), (C.c_int * 3)(*hi), rhoY, rhoH, T,
+                                                     mask, rhoYdot, _stream(stream)))
 
     def stats(self) -> dict:
         st = MiStats()

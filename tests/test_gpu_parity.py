@@ -53,12 +53,15 @@ def test_rates_match_oracle(wl, mech):
     d = cells.generate_workload(wl, ncell=2048)
     m = O.Mech(mech)
     r = _mk(mech)
-    rY, rE, T = _dev(d["rY"]), _dev(d["rE"]), _dev(d["T"] + 15.0)   # perturbed guess for HY2T
+    T_in = d["T"] + 15.0   # perturbed guess for HY2T
+    rY, rE, T = _dev(d["rY"]), _dev(d["rE"]), _dev(T_in)
     out = torch.zeros_like(rY)
-    r.reaction_rates(rY, rE, T, None, out)
+    T_out = torch.zeros_like(T)
+    r.reaction_rates(rY, rE, T, None, out, T_out=T_out)
     torch.cuda.synchronize()
     got = out.cpu().numpy()
-    Tg = T.cpu().numpy()
+    Tg = T_out.cpu().numpy()
+    np.testing.assert_array_equal(T.cpu().numpy(), T_in)   # T is read only
     for c in range(0, 2048, 7):
         rho = d["rY"][:, c].sum()
         Y = d["rY"][:, c] / rho
@@ -68,6 +71,57 @@ def test_rates_match_oracle(wl, mech):
         scale = np.abs(w).max()
         assert np.max(np.abs(got[:, c] - w)) <= 1e-11 * scale + 1e-300, c
     r.close()
+
+
+@pytest.mark.parametrize("wl,mech", [("drm19_64", "drm19"), ("dodecane53_64_mix", "dodecane53")])
+def test_reaction_rates_box_in_pelelm_units(wl, mech):
+    """compute_instantaneous_reaction_rates on an Array4 box inside a larger fab with
+    ghost cells and an EB mask, MKS in and out (reactionRateRhoY, Source/PeleLM_K.H:57-98):
+    the oracle's HY2T and wdot with PeleLM's unit factors restated around them, T left
+    unchanged, covered cells zero, ghost cells untouched."""
+    n_side, g = 8, 2
+    n = n_side ** 3
+    d = cells.generate_workload(wl, ncell=n)
+    m = O.Mech(mech)
+    N = d["rY"].shape[0]
+    F = n_side + 2 * g
+    lo, hi = (3, -2, 5), (3 + n_side - 1, -2 + n_side - 1, 5 + n_side - 1)
+    flo = tuple(x - g for x in lo)
+    inner = (slice(None), slice(g, g + n_side), slice(g, g + n_side), slice(g, g + n_side))
+    # MKS state: rhoY x 1e3 (kg/m^3), rhoH x 0.1 (J/m^3), T + 10 K as the HY2T guess
+    rhoY = np.full((N, F, F, F), 7.0)
+    rhoH = np.full((1, F, F, F), 7.0)
+    Tf = np.full((1, F, F, F), 7.0)
+    rhoY[inner] = (d["rY"] * 1e3).reshape(N, n_side, n_side, n_side)
+    rhoH[inner] = (d["rE"] * 0.1).reshape(1, n_side, n_side, n_side)
+    Tf[inner] = (d["T"] + 10.0).reshape(1, n_side, n_side, n_side)
+    mask = np.ones((1, F, F, F), dtype=np.int32)
+    rng = np.random.default_rng(7)
+    covered = rng.random((n_side, n_side, n_side)) < 0.2
+    mask[inner][0][covered] = -1
+    mflat = np.where(covered.reshape(-1), -1, 1)
+    gY, gH, gT, gM = _dev(rhoY), _dev(rhoH), _dev(Tf), _dev(mask, torch.int32)
+    gR = _dev(np.full((N, F, F, F), -3.0))
+    r = _mk(mech)
+    r.reaction_rates_box(lo, hi, reactor.array4(gY, flo, None, N), reactor.array4(gH, flo, None, 1),
+                         reactor.array4(gT, flo, None, 1), reactor.int4(gM, flo),
+                         reactor.array4(gR, flo, None, N))
+    torch.cuda.synchronize()
+    r.close()
+    got = gR.cpu().numpy()
+    np.testing.assert_array_equal(gT.cpu().numpy(), Tf)            # T read only
+    outside = np.ones(got.shape, dtype=bool)
+    outside[inner] = False
+    assert np.all(got[outside] == -3.0)                            # ghost cells untouched
+    gin = got[inner].reshape(N, -1)
+    assert np.all(gin[:, mflat < 0] == 0.0)
+    for c in np.nonzero(mflat > 0)[0][::3]:
+        rho_mks = d["rY"][:, c].sum() * 1e3
+        Y = d["rY"][:, c] * 1e3 / rho_mks
+        Tc, _ = m.hy2t(d["rE"][c] * 0.1 / rho_mks * 1e4, Y, d["T"][c] + 10.0)
+        w = m.wdot(rho_mks * 1e-3 * Y / m.W, Tc) * m.W * 1e3
+        scale = np.abs(w).max()
+        assert np.max(np.abs(gin[:, c] - w)) <= 1e-10 * scale + 1e-300, c
 
 
 @pytest.mark.parametrize("wl,mech,n", [("lidryer_16", "LiDryer", 4096), ("drm19_64", "drm19", 4096),
@@ -323,3 +377,22 @@ def test_one_lane_linear_algebra_matches_oracle(wl, mech, lu):
     else:
         _parity(f"{wl} lane fp32_ir", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref,
                 y_frac=1.0, dT=1e-3, fc=0.95)
+
+
+def test_one_lane_solves_in_graph_trips_cover_every_cell():
+    """mi.lu_form = lane with graph-captured trips on more cells than one fixed solve grid
+    of 16,384 lanes holds: the one-lane solve's grid is sized for every cell, so cells
+    past that index get their Newton solves (a grid of 256 blocks left them with stale
+    solutions)."""
+    n = 32768
+    d = cells.generate_workload("lidryer_16", ncell=n)
+    ref = O.Mech("LiDryer").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    r = _mk("LiDryer", ode__rtol=1e-6, ode__atol=1e-10, mi__lu_form="lane", mi__graph_trips=8)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    assert st["nfail"] == 0 and st["graph_trips_used"] == 8
+    _parity("lidryer 32768 lane graphs", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)

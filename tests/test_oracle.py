@@ -160,6 +160,46 @@ def test_bdf_matches_radau_fixture(wl):
     assert np.max(np.abs(loose["T"] - ref[N]) / ref[N]) < 1e-5
 
 
+@pytest.mark.parametrize("wl", ["lidryer_16", "drm19_64", "drm19_128_mix", "dodecane53_64_mix"])
+def test_radau_matches_scipy_radau_fixture(wl):
+    """The C++ Radau IIA-5 (oracle/radau5.cpp) against SciPy's Radau on the same cells:
+    two implementations of the method, one of them not this project's."""
+    with open(os.path.join(HERE, "golden", f"radau_{wl}.json")) as f:
+        g = json.load(f)
+    m = O.Mech(g["mechanism"])
+    N = m.nspec
+    y0 = np.array([c["y0"] for c in g["cells"]]).T
+    rE0 = np.array([c["rE0"] for c in g["cells"]])
+    ref = np.array([c["y_end"] for c in g["cells"]]).T
+    out = m.radau_react(y0[:N], y0[N], rE0, g["dt"], rtol=1e-12, atol=1e-20)
+    assert out["nfail"] == 0
+    np.testing.assert_allclose(out["T"], ref[N], rtol=1e-9)
+    assert np.max(np.abs(out["rY"] - ref[:N]) / (1e-8 * np.abs(ref[:N]) + 1e-18)) < 1.0
+
+
+@pytest.mark.parametrize("wl", ["lidryer_16", "drm19_128_mix", "dodecane53_64_mix"])
+def test_radau_and_bdf_agree_with_forcing(wl):
+    """The two integrators of the oracle (CVODE-algorithm BDF and Radau IIA-5) at tight
+    tolerance on forced cells agree to 1e-9 relative, and both return a T consistent with
+    their own (rho h, rho Y): T_out = HY2T(rhoh_out / rho_out, Y_out)."""
+    n = 12
+    d = cells.generate_workload(wl, ncell=n)
+    m = O.Mech(d["mech"])
+    rng = np.random.default_rng(11)
+    fY = rng.normal(size=d["rY"].shape) * 1e-4 * d["rY"] / d["dt"]
+    fH = rng.normal(size=n) * 1e-4 * np.abs(d["rE"]) / d["dt"]
+    a = m.radau_react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-12, atol=1e-20, rYsrc=fY, rEsrc=fH)
+    b = m.react(d["rY"], d["T"], d["rE"], d["dt"], rYsrc=fY, rEsrc=fH, rtol=1e-12, atol=1e-20, max_order=5)
+    assert a["nfail"] == 0 and b["nfail"] == 0
+    np.testing.assert_allclose(a["T"], b["T"], rtol=1e-10)
+    assert np.max(np.abs(a["rY"] - b["rY"]) / (1e-9 * np.abs(b["rY"]) + 1e-18)) < 1.0
+    for out in (a, b):
+        for c in range(n):
+            rho = out["rY"][:, c].sum()
+            Tc, _ = m.hy2t(out["rE"][c] / rho, out["rY"][:, c] / rho, out["T"][c])
+            assert abs(Tc - out["T"][c]) < 1e-6
+
+
 def test_react_conserves_mass_and_energy_with_forcing():
     d = cells.generate_workload("drm19_64", ncell=256)
     m = O.Mech("drm19")
