@@ -9,20 +9,43 @@ and is inside the timed region; it moves 0.4 GB, < 1 % of a step).
 
 Forms measured (each is a complete, independently parity-tested reactor), named
 schedule[+modifier...]:
-  plain  -- FP64 BDF, fixed cell-to-lane mapping: finished cells idle until the box is done
-  queue  -- FP64 BDF, compacted work lists: finished cells leave the pipeline
-  auto   -- the schedule (plain/queue, with or without graph-captured trips) the reactor's
-            autotuner picks on its first call for the box size
+  plain      -- FP64 BDF, fixed cell-to-lane mapping, cells ordered by estimated cost
+                (mi.order = rate, the default: cells sorted by their initial rates'
+                tolerance-scaled change over dt, so that a wave's cells finish together)
+  queue      -- FP64 BDF, compacted work lists: finished cells leave the pipeline
+  persistent -- FP64 BDF in one persistent kernel per call: waves keep their cells through
+                all trips and claim new cells from a device counter (no host round trips)
+  auto       -- the schedule (plain/queue, with or without graph-captured trips) the
+                reactor's autotuner picks on its first call for the box size
+  +unordered -- no cost ordering (mi.order = none): cell c in workspace slot c, so a
+            wave's finished cells idle until its slowest cell is done
+  +fc    -- cost ordering by the RHS counts the FC array holds on entry (mi.order = fc:
+            the previous step's FuncCount, which PeleLM keeps between steps; here the
+            previous timed call's, on the same cells)
   +fp32  -- FP32 LU of I - gamma J with up to 3 FP64-residual refinement sweeps
+  +ir0   -- with +fp32: no refinement sweeps (the Newton iteration's own FP64 residuals
+            correct the FP32 solves)
   +graphs -- trips captured into HIP graphs of 8
   +pred  -- the learned Newton predictor, trained before the timed region on pairs the
             plain integrator generates from cells of a seed of their own (setup time
             reported); it runs on the queue schedule without graph capture
-The default set is the plain FP64 form and the chosen form (queue+fp32+pred).
 `value` is the fastest form's cells/s; every form reports its max |dT| against the
-plain form on all cells.  The CPU baseline is this project's CPU oracle
-(CVODE-algorithm restatement) on a bounded sample of the same cells, on the host
-threads OMP_NUM_THREADS allows.
+first form (the drop-in FP64 form) on all cells, and its error against a tight FP64
+oracle (rtol 1e-12, BDF order 5) on a seeded sample of the box.
+
+Roofline (the measurement the survey defines for this path): the bound is FP64
+arithmetic.  Algorithmic flops per cell come from the integrator's counters times the
+generator's static operation counts (RHS, Jacobian), (2/3) n^3 + 2 n^2 per
+factorisation, 2 n^2 per Newton solve and the Nordsieck step updates; `achieved` is
+that work per launch of the kernel with the largest device time over its mean launch
+time (HIP events on the reactor's stream), against MI355X's 78.6 TFLOP/s FP64 peak.
+The whole advance's FP64 rate is reported beside it, and `traffic` / `traffic_ratio`
+give that kernel's and the whole advance's measured HBM bytes (rocprofv3 byte counters,
+profiles/) against the algorithmic (3N + 6) * 8 + 4 bytes per cell of one advance.
+
+The CPU baseline is this project's CPU oracle (CVODE-algorithm restatement; the
+reference's own reactor is not in its tree) on a bounded sample of the same cells, on
+the host threads OMP_NUM_THREADS allows, plus a one-thread run for cells/s per core.
 """
 from __future__ import annotations
 
@@ -46,12 +69,16 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def mech_ops(mech: str):
+    with open(os.path.join(ROOT, "pelelm_amd", "mechanisms", f"{mech}_ops.json")) as f:
+        return json.load(f)
+
+
 def rhs_ops(mech: str):
     """Algorithmic flops of one right-hand-side evaluation (fixed part, per HY2T
     iteration) and its transcendentals, from the generator's static counts."""
-    with open(os.path.join(ROOT, "pelelm_amd", "mechanisms", f"{mech}_ops.json")) as f:
-        d = json.load(f)["functions"]
-    N = json.load(open(os.path.join(ROOT, "pelelm_amd", "mechanisms", f"{mech}_ops.json")))["nspec"]
+    ops = mech_ops(mech)
+    d, N = ops["functions"], ops["nspec"]
     cph = d["thermo_cph"]["flops"]
     # rho, 1/rho, Y; after HY2T: log T, 1/T, C, gibbs, wdot, cp/h, f, rho cp, h.wdot, fT
     fixed = (2 * N + 1) + 2 + N + d["gibbs"]["flops"] + d["wdot"]["flops"] + cph + 7 * N + 5
@@ -60,95 +87,114 @@ def rhs_ops(mech: str):
     return fixed, per_iter, trans
 
 
+def bytes_per_cell(nspec: int) -> int:
+    """Algorithmic HBM bytes of one advance of one cell: rhoY, T, rhoh read and written,
+    rhoY forcing, rhoh forcing and FC (8 bytes each), the mask (4 bytes)."""
+    return (3 * nspec + 6) * 8 + 4
+
+
 LANE_WORDS = 53   # 8-byte words of the integrator's stored lane state (independent of the mechanism)
 
 
 def rhs_bytes(N: int) -> int:
-    """HBM bytes one k_eval cell-evaluation moves: zn0, zn1, acor, forcing and the lane
-    state read, y, f and T written."""
+    """Workspace bytes one k_eval cell-evaluation moves: zn0, zn1, acor, forcing and the
+    lane state read, y, f and T written."""
     N1 = N + 1
     return 8 * (3 * N1 + N + LANE_WORDS + 2 * N1 + 1)
 
 
 KINDS = ["k_eval", "k_setup", "k_consume", "k_lu_coop", "k_predict", "k_solve_coop"]
 KIND_DESC = {"k_eval": "right-hand side (HY2T, rates, energy)", "k_setup": "analytic Jacobian + set-up",
-             "k_consume": "Newton update (LU solve) and BDF step logic", "k_lu_coop": "LU of I - gamma J",
-             "k_predict": "learned Newton predictor", "k_solve_coop": "Newton linear solves (substitution)"}
+             "k_consume": "Newton update and BDF step logic", "k_lu_coop": "LU of I - gamma J",
+             "k_predict": "learned Newton predictor", "k_solve_coop": "Newton linear solves (substitution)",
+             "k_persist": "the whole integration in one persistent kernel"}
 
 
 def kernel_model(st, nspec, mech):
-    """Algorithmic work of each kernel kind over the call: (flops, hbm bytes)."""
+    """Algorithmic work of each kernel kind over the call: (FP64 flops, workspace bytes).
+    The Newton update and step logic count 8 n per iteration (update, WRMS norm) and
+    2 * 5n * 3 per step (Nordsieck predict, correct, error weights)."""
     N, n = nspec, nspec + 1
     fixed, per_iter, _ = rhs_ops(mech)
-    with open(os.path.join(ROOT, "pelelm_amd", "mechanisms", f"{mech}_ops.json")) as f:
-        jac_flops = json.load(f)["functions"]["jacobian"]["flops"]
-    evals = st["kernel_cells"][0]
-    nni, nst, nsetups, nje = st["nni"], st["nsteps"], st["nsetups"], st["nje"]
-    consume_passes = st["kernel_cells"][2]
+    jac_flops = mech_ops(mech)["functions"]["jacobian"]["flops"]
+    nfe, nni, nst, nsetups, nje = st["nfe"], st["nni"], st["nsteps"], st["nsetups"], st["nje"]
+    evals = st["kernel_cells"][0] or nfe
+    consume_passes = st["kernel_cells"][2] or nfe
+    nir = st.get("nlu_refine", 0)
     return {
-        "k_eval": (evals * fixed + st["nhy2t_iters"] * per_iter, evals * rhs_bytes(N)),
+        "k_eval": (nfe * fixed + st["nhy2t_iters"] * per_iter, evals * rhs_bytes(N)),
         # Jacobian where evaluated; M = I - gamma J formation is in the LU kernel
         "k_setup": (nje * jac_flops, nje * 8 * (n * n + 2 * n + 2 * LANE_WORDS)),
-        # Newton update, convergence test and WRMS norms; step logic per step: Nordsieck
-        # predict/correct over (q+1) n with q <= 4
         "k_consume": (nni * 8 * n + nst * 2 * 5 * n * 3,
                       consume_passes * 8 * (LANE_WORDS * 2 + 3 * n) + nni * 8 * 3 * n + nst * 8 * 2 * 5 * n),
-        # forward/back substitution with the factors (n^2 words) and B, X vectors
-        "k_solve_coop": (nni * 2 * n * n, nni * 8 * (n * n + 2 * n)),
-        "k_lu_coop": (nsetups * (2 * n ** 3 // 3 + 2 * n * n), nsetups * 8 * 2 * n * n),
+        # forward/back substitution, plus per refinement sweep a residual and a second solve
+        "k_solve_coop": (nni * 2 * n * n + nir * 4 * n * n, nni * 8 * (n * n + 2 * n)),
+        # I - gamma J (n^2) and the factorisation
+        "k_lu_coop": (nsetups * (2 * n ** 3 // 3 + 2 * n * n + n * n), nsetups * 8 * 2 * n * n),
         "k_predict": (st["npred_used"] * 2 * ((N + 3) * 128 + 2 * 128 * 128 + 128 * n), st["npred_used"] * 8 * 3 * n),
     }
 
 
-def measured_traffic(kernel: str, form: str, workload: str):
-    """HBM bytes per launch of `kernel` measured by the byte counters (FETCH_SIZE and
-    WRITE_SIZE, each scaled by what it reads for a copy of known size), from the
-    committed summary for this workload; None when the summary has no such row."""
+def advance_flops(st, nspec, mech):
+    return float(sum(f for f, _ in kernel_model(st, nspec, mech).values()))
+
+
+def profile_table(workload: str, form: str):
+    """Rows of the committed byte-counter summary for this workload and form:
+    {kernel: (launches, bytes per launch)}, or None when there is no summary."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_{form}.md")
     if not os.path.exists(path):
         return None
+    rows = {}
     for line in open(path):
         cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0] == kernel:
+        if len(cells) >= 4:
             try:
-                return float(cells[3])
+                rows[cells[0]] = (int(cells[1]), float(cells[3]))
             except ValueError:
-                return None
-    return None
+                pass
+    return rows
 
 
-def kernel_roofline(st, nspec, mech, workload, traffic_bytes=None, form="plain"):
-    """Roofline of the kernel with the largest device time: algorithmic work per launch
-    over its mean launch time (HIP events on the reactor's stream), against FP64 peak
-    (the vector and matrix FP64 peaks of MI355X coincide) or HBM peak for k_consume."""
-    kms = st["kernel_ms"]
-    dom = int(np.argmax(kms))
-    name = KINDS[dom]
-    launches = max(1, st["kernel_launches"][dom])
-    avg_ms = kms[dom] / launches
+def kernel_roofline(st, nspec, mech, workload, ncell, form):
+    """The FP64 roofline of the kernel with the largest device time, and the whole
+    advance's FP64 rate and HBM traffic."""
     model = kernel_model(st, nspec, mech)
-    flops, byts = model[name]
-    per_launch_flops, per_launch_bytes = flops / launches, byts / launches
-    if name == "k_consume":
-        achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-        line = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS}
-    else:
-        achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
-        line = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS}
-    line["traffic"] = traffic_bytes if traffic_bytes is not None else measured_traffic(name, form, workload)
-    line.update({
+    total_flops = advance_flops(st, nspec, mech)
+    kms = st["kernel_ms"]
+    if sum(kms) > 0:
+        dom = int(np.argmax(kms))
+        name = KINDS[dom]
+        launches = max(1, st["kernel_launches"][dom])
+        avg_ms = kms[dom] / launches
+        per_launch_flops = model[name][0] / launches
+    else:   # one kernel runs the whole integration
+        name, launches, avg_ms, per_launch_flops = "k_persist", 1, st["ms_kernel"], total_flops
+    achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
+    table = profile_table(workload, form) or {}
+    traffic = table.get(name, (None, None))[1]
+    measured_total = sum(l * b for l, b in table.values()) if table else None
+    algo_total = bytes_per_cell(nspec) * ncell
+    line = {
+        "bound": "fp64",
+        "bound_note": "FP64 arithmetic on the vector ALUs (the path's bound; MI355X's FP64 vector and "
+                      "matrix peaks are both 78.6 TFLOP/s, and this kernel issues no MFMA)",
+        "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+        "traffic": traffic,
         "kernel": f"{name}<mech_{mech}>: {KIND_DESC[name]}",
         "avg_launch_ms": avg_ms, "launches": launches,
-        "algorithmic_flops_per_launch": per_launch_flops, "algorithmic_bytes_per_launch": per_launch_bytes,
+        "algorithmic_flops_per_launch": per_launch_flops,
+        "advance_fp64_flops": total_flops,
+        "advance_fp64_tflops": total_flops / (st["ms_kernel"] * 1e-3) / 1e12,
+        "advance_fp64_frac": total_flops / (st["ms_kernel"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+        "algorithmic_bytes_per_advance": algo_total,
+        "measured_bytes_per_advance": measured_total,
+        "traffic_ratio": (measured_total / algo_total) if measured_total else None,
         "kernel_ms": {KINDS[i]: kms[i] for i in range(len(KINDS))},
         "kernel_launches": {KINDS[i]: st["kernel_launches"][i] for i in range(len(KINDS))},
-        "kernel_tflops": {k: (model[k][0] / (kms[i] * 1e-3) / 1e12 if kms[i] > 0 else None)
-                          for i, k in enumerate(KINDS)},
-        "kernel_GBs": {k: (model[k][1] / (kms[i] * 1e-3) / 1e9 if kms[i] > 0 else None)
-                       for i, k in enumerate(KINDS)},
-    })
+        "kernel_fp64_tflops": {k: (model[k][0] / (kms[i] * 1e-3) / 1e12 if kms[i] > 0 else None)
+                               for i, k in enumerate(KINDS)},
+    }
     return line
 
 
@@ -168,13 +214,19 @@ def run_form(form, mech, dev_in, ncell, nspec, dt, steps, warmup, profile, net=N
         r.set("mi.ir_sweeps", 3)
     if "graphs" in mods:
         r.set("mi.graph_trips", 8)
+    if "unordered" in mods:
+        r.set("mi.order", "none")
+    if "fc" in mods:
+        r.set("mi.order", "fc")
+    if "ir0" in mods:
+        r.set("mi.ir_sweeps", 0)
     r.init(2, ncell)
     if "pred" in mods:
         predictor.attach(r, net)
     rY = torch.empty_like(dev_in["rY"])
     T = torch.empty_like(dev_in["T"])
     rE = torch.empty_like(dev_in["rE"])
-    FC = torch.zeros(ncell, dtype=torch.float64, device="cuda")
+    FC = torch.zeros(ncell, dtype=torch.float64, device="cuda")   # kept between calls (FuncCount)
 
     def step():
         rY.copy_(dev_in["rY"])
@@ -203,6 +255,40 @@ def run_form(form, mech, dev_in, ncell, nspec, dt, steps, warmup, profile, net=N
     return out
 
 
+def error_vs_tight(d, sample, tight, res, nspec):
+    """A form's error against the tight FP64 oracle on the sampled cells: per species the
+    max and RMS relative error over the cells where that species' mass fraction is above
+    1e-6 (below it the rtol 1e-6 integration does not control its relative error), max
+    |dT|, and the max error in units of the integration's own tolerance."""
+    gY = res["rY"][:, sample].cpu().numpy()
+    gT = res["T"][sample].cpu().numpy()
+    tY, tT = tight["rY"], tight["T"]
+    Yfrac = tY / tY.sum(axis=0, keepdims=True)
+    rel = np.abs(gY - tY) / np.abs(tY)
+    per_max, per_rms = [], []
+    for k in range(nspec):
+        sel = Yfrac[k] > 1e-6
+        per_max.append(float(rel[k, sel].max()) if sel.any() else None)
+        per_rms.append(float(np.sqrt(np.mean(rel[k, sel] ** 2))) if sel.any() else None)
+    tol = 1e-6 * np.abs(tY) + 1e-10
+    return {"cells": int(len(sample)),
+            "max_abs_dT_K": float(np.abs(gT - tT).max()),
+            "max_rel_err_species": max(v for v in per_max if v is not None),
+            "rms_rel_err_species": float(np.sqrt(np.mean([v ** 2 for v in per_rms if v is not None]))),
+            "per_species_max_rel": per_max, "per_species_rms_rel": per_rms,
+            "max_err_over_tol": float((np.abs(gY - tY) / tol).max())}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,13 +296,12 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="drm19_128_mix")
     ap.add_argument("--ncell", type=int, default=None, help="override the cell count (default: the workload's)")
-    ap.add_argument("--forms", default="plain,queue+fp32+pred")
-    ap.add_argument("--pred-pairs", type=int, default=131072, help="training pairs for +pred forms")
+    ap.add_argument("--forms", default="plain,plain+unordered,plain+fc,plain+fp32,queue+fp32+pred,persistent")
+    ap.add_argument("--pred-pairs", type=int, default=1_000_000, help="training pairs for +pred forms")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample duration")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tight-cells", type=int, default=4096, help="sampled cells for the tight-oracle errors")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the tight-oracle errors")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
-    ap.add_argument("--traffic-bytes", type=float, default=None,
-                    help="measured HBM bytes per launch of the dominant kernel (from the byte counters)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only copy a device buffer of this many MiB 5 times (calibrates HBM byte counters)")
     args = ap.parse_args(argv)
@@ -261,7 +346,7 @@ def main(argv=None):
         forms[form] = run_form(form, d["mech"], dev_in, ncell, nspec, d["dt"], args.steps, args.warmup,
                                profile=not args.no_profile, net=net)
         log(f"{form}: {forms[form]['cells_per_s']:.4g} cells/s, {forms[form]['ms_per_step']:.1f} ms/step")
-    ref_form = "plain" if "plain" in forms else next(iter(forms))
+    ref_form = next(iter(forms))
     T_ref = forms[ref_form]["result"]["T"]
     for f, v in forms.items():
         v["dT_vs_ref"] = float((v["result"]["T"] - T_ref).abs().max().item())
@@ -271,12 +356,22 @@ def main(argv=None):
 
     roofline = None
     if not args.no_profile:
-        roofline = kernel_roofline(st, nspec, d["mech"], args.workload, traffic_bytes=args.traffic_bytes, form=best)
+        prof_form = best.replace("+", "_")
+        roofline = kernel_roofline(st, nspec, d["mech"], args.workload, ncell, prof_form)
 
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
         m = O.Mech(d["mech"])
+        # errors against the tight FP64 oracle on a seeded sample of the box
+        sample = np.sort(np.random.default_rng(2024).choice(ncell, size=min(ncell, args.tight_cells),
+                                                            replace=False))
+        t0 = time.perf_counter()
+        tight = m.react(d["rY"][:, sample], d["T"][sample], d["rE"][sample], d["dt"], rtol=1e-12, atol=1e-20,
+                        max_order=5, max_steps=200000)
+        log(f"tight oracle: {len(sample)} cells in {time.perf_counter() - t0:.1f}s, {tight['nfail']} failed")
+        for f, v in forms.items():
+            v["err_vs_tight"] = error_vs_tight(d, sample, tight, v["result"], nspec)
         nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         n0 = min(ncell, 16384)
         t0 = time.perf_counter()
@@ -291,12 +386,19 @@ def main(argv=None):
             rate = n1 / (time.perf_counter() - t0)
         else:
             n1 = n0
-        log(f"cpu oracle: {n1} cells at {rate:.4g} cells/s on {nthr} threads")
+        n_one = min(ncell, 2048)
+        t0 = time.perf_counter()
+        m.react(d["rY"][:, :n_one], d["T"][:n_one], d["rE"][:n_one], d["dt"], rtol=1e-6, atol=1e-10,
+                max_steps=20000, nthreads=1)
+        rate_one = n_one / (time.perf_counter() - t0)
+        log(f"cpu oracle: {n1} cells at {rate:.4g} cells/s on {nthr} threads; {rate_one:.4g} cells/s on one")
         pf = forms[ref_form]   # the drop-in FP64 form is the one held to the oracle
         gT = pf["result"]["T"][:n1].cpu().numpy()
         gY = pf["result"]["rY"][:, :n1].cpu().numpy()
         tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
         cpu = {"value": rate, "unit": "cells/s", "cores": nthr, "kind": "port",
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+               "per_core": {"value": rate_one, "unit": "cells/s/core", "sample": f"first {n_one} cells, 1 thread"},
                "sample": f"first {n1} cells of the same {args.workload} box (same seed, dt, tolerances); "
                          f"CVODE-algorithm FP64 BDF restatement, OpenMP dynamic schedule",
                "gpu_form_compared": ref_form,
@@ -320,6 +422,7 @@ def main(argv=None):
         "forms": {f: {"cells_per_s": v["cells_per_s"], "ms_per_step": v["ms_per_step"], "nfail": v["nfail"],
                       f"max_abs_dT_vs_{ref_form}_K": v["dT_vs_ref"],
                       "speedup_vs_" + ref_form: v["cells_per_s"] / forms[ref_form]["cells_per_s"],
+                      "err_vs_tight": v.get("err_vs_tight"),
                       "schedule_used": v["stats"].get("schedule_used"),
                       "graph_trips_used": v["stats"].get("graph_trips_used"),
                       "newton_iters_per_cell": v["stats"]["nni"] / max(1, v["stats"]["ncells"]),
@@ -329,6 +432,8 @@ def main(argv=None):
                       "rhs_per_cell": v["stats"]["nfe"] / max(1, v["stats"]["ncells"]),
                       "lu_per_cell": v["stats"]["nsetups"] / max(1, v["stats"]["ncells"]),
                       "trips": v["stats"]["ntrips"], "device_ms": v["stats"]["ms_kernel"],
+                      "advance_fp64_tflops": advance_flops(v["stats"], nspec, d["mech"])
+                                             / (v["stats"]["ms_kernel"] * 1e-3) / 1e12,
                       "kernel_ms": dict(zip(KINDS, v["stats"]["kernel_ms"]))}
                   for f, v in forms.items()},
         "predictor_setup": pred_setup,

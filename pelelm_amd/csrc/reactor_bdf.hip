@@ -1089,6 +1089,11 @@ __device__ __forceinline__ void cell_ijk(const ReactArgs& a, long c, int& i, int
   j = (int)((c / a.nx) % a.ny);
   k = (int)(c / ((long)a.nx * a.ny));
 }
+// the caller's cell that workspace slot s integrates (cost ordering), and its (i, j, k)
+__device__ __forceinline__ long slot_cell(const ReactArgs& a, long s) { return a.slot_cell ? a.slot_cell[s] : s; }
+__device__ __forceinline__ void slot_ijk(const ReactArgs& a, long s, int& i, int& j, int& k) {
+  cell_ijk(a, slot_cell(a, s), i, j, k);
+}
 
 template <class M>
 __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
@@ -1098,7 +1103,7 @@ __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
   bool active = false;
   if (c < a.ncell) {
     int ci, cj, ck;
-    cell_ijk(a, c, ci, cj, ck);
+    slot_ijk(a, c, ci, cj, ck);
     const int mk = a.mask.p ? a.mask.p[cell_off(a.mask, ci, cj, ck)] : 1;
     const WS w = cell_ws<M>(a.ws, c);
     Lane<M> st;
@@ -1119,7 +1124,7 @@ __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
       st.setup_pending = 0;
       st.lu64 = 0;
       st.pred_state = 0;
-      if (a.cell_status) a.cell_status[c] = 0;
+      if (a.cell_status) a.cell_status[slot_cell(a, c)] = 0;
     } else {
 #pragma unroll
       for (int i = 0; i < N1; ++i) w[S::ATOL + i] = a.atol[i];
@@ -1801,7 +1806,7 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_co
         }
       }
       int ci, cj, ck;
-      cell_ijk(a, c, ci, cj, ck);
+      slot_ijk(a, c, ci, cj, ck);
       if (reached) {
         // CV_NORMAL output: interpolate the Nordsieck array back to tout
         const double sfac = (st.tout - st.tn) / st.h;
@@ -1825,7 +1830,7 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_co
       if (st.phase == PH_NEED_CELL) {
         const bool ok = (st.status == ST_OK);
         a.FC.p[cell_off(a.FC, ci, cj, ck)] = ok ? (double)st.nfe : -(double)st.nfe;
-        if (a.cell_status) a.cell_status[c] = st.status;
+        if (a.cell_status) a.cell_status[slot_cell(a, c)] = st.status;
         t_cells = 1;
         t_nst = st.nst; t_nfe = st.nfe; t_nje = st.nje; t_nsetups = st.nsetups; t_nni = st.nni;
         t_ncfn = st.ncfn; t_netf = st.netf; t_fail = ok ? 0 : 1; t_maxst = st.nst;
@@ -1882,10 +1887,10 @@ __global__ void __launch_bounds__(64) k_abandon(const ReactArgs a, const int* __
     load_lane(st, a, c);
     if (st.phase != PH_NEED_CELL) {
       int ci, cj, ck;
-      cell_ijk(a, c, ci, cj, ck);
+      slot_ijk(a, c, ci, cj, ck);
       st.finish(ST_TOO_MUCH_WORK);
       a.FC.p[cell_off(a.FC, ci, cj, ck)] = -(double)st.nfe;
-      if (a.cell_status) a.cell_status[c] = ST_TOO_MUCH_WORK;
+      if (a.cell_status) a.cell_status[slot_cell(a, c)] = ST_TOO_MUCH_WORK;
       store_lane(st, w);
       failed = true;
     }
@@ -2282,6 +2287,57 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) rates_kernel(const Ar
   M::wdot(C, Tc, logT, invT, g, wd);
 #pragma unroll
   for (int n = 0; n < N; ++n) rdot.p[od + n * rdot.ns] = wd[n] * M::mw(n) * oscale;
+}
+
+// Cost-ordering keys (mi.order), one per caller cell c: the float bits of a cost estimate
+// (non-negative floats order like their bit patterns) and c itself as the value to sort.
+//   ORDER_FC:   |FC| as the caller's array holds it -- the cell's RHS count of the previous
+//               step, which PeleLM keeps between steps as FuncCount (0: unknown; the
+//               ordering changes only which lanes run which cells, so any content is safe)
+//   ORDER_RATE: max_k |W_k wdot_k + F_Y,k| dt / (rtol rhoY_k + atol_k) at the input state,
+//               T from HY2T(rhoH / rho, Y): how many tolerance units the fastest-moving
+//               species changes over dt
+// Cells with mask < 0 get key 0.
+template <class M>
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(4) k_cost_key(const ReactArgs a, const int mode,
+                                                                      unsigned int* __restrict__ keys,
+                                                                      int* __restrict__ vals) {
+  constexpr int N = M::NS;
+  const long c = (long)blockIdx.x * 64 + threadIdx.x;
+  if (c >= a.ncell) return;
+  int i, j, k;
+  cell_ijk(a, c, i, j, k);
+  float key = 0.0f;
+  const bool masked = a.mask.p && a.mask.p[cell_off(a.mask, i, j, k)] < 0;
+  if (!masked && mode == ORDER_FC) {
+    const double fc = fabs(a.FC.p[cell_off(a.FC, i, j, k)]);   // any value orders: NaN and inf sort last
+    key = isfinite(fc) ? (float)fmin(fc, 3.0e38) : 3.0e38f;
+  } else if (!masked && mode == ORDER_RATE) {
+    const long oy = cell_off(a.rY, i, j, k);
+    double y[N], Y[N], rho = 0.0;
+#pragma unroll
+    for (int n = 0; n < N; ++n) { y[n] = a.rY.p[oy + n * a.rY.ns]; rho += y[n]; }
+    const double irho = 1.0 / rho;
+#pragma unroll
+    for (int n = 0; n < N; ++n) Y[n] = y[n] * irho;
+    const double T = hy2t<M>(a.rE.p[cell_off(a.rE, i, j, k)] * irho, Y, a.T.p[cell_off(a.T, i, j, k)]);
+    const double logT = log(T), invT = 1.0 / T;
+    double C[N], g[N], wd[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) C[n] = y[n] * M::imw(n);
+    M::gibbs(T, logT, g);
+    M::wdot(C, T, logT, invT, g, wd);
+    double kmax = 0.0;
+    const long of = a.rYsrc.p ? cell_off(a.rYsrc, i, j, k) : 0;
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      const double fy = wd[n] * M::mw(n) + (a.rYsrc.p ? a.rYsrc.p[of + n * a.rYsrc.ns] : 0.0);
+      kmax = fmax(kmax, fabs(fy) * a.dt / (a.rtol * fabs(y[n]) + a.atol[n]));
+    }
+    key = isfinite(kmax) ? (float)fmin(kmax, 3.0e38) : 3.0e38f;
+  }
+  keys[c] = __float_as_uint(key);
+  vals[c] = (int)c;
 }
 
 __global__ void counters_reset_kernel(unsigned long long* counters) {
@@ -2685,6 +2741,12 @@ hipError_t launch_rates_t(const RatesArgs& ra, hipStream_t stream) {
   return hipGetLastError();
 }
 
+template <class M>
+hipError_t launch_cost_keys_t(const ReactArgs& a, int mode, unsigned int* keys, int* vals, hipStream_t stream) {
+  hipLaunchKernelGGL(k_cost_key<M>, dim3((unsigned)((a.ncell + 63) / 64)), dim3(64), 0, stream, a, mode, keys, vals);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 int mech_nspec(int mech) {
@@ -2772,6 +2834,16 @@ hipError_t launch_rates(int mech, const RatesArgs& ra, hipStream_t stream) {
     case MECH_LIDRYER: return launch_rates_t<mech_LiDryer>(ra, stream);
     case MECH_DRM19: return launch_rates_t<mech_drm19>(ra, stream);
     case MECH_DODECANE53: return launch_rates_t<mech_dodecane53>(ra, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_cost_keys(int mech, const ReactArgs& a, int mode, unsigned int* keys, int* vals,
+                            hipStream_t stream) {
+  switch (mech) {
+    case MECH_LIDRYER: return launch_cost_keys_t<mech_LiDryer>(a, mode, keys, vals, stream);
+    case MECH_DRM19: return launch_cost_keys_t<mech_drm19>(a, mode, keys, vals, stream);
+    case MECH_DODECANE53: return launch_cost_keys_t<mech_dodecane53>(a, mode, keys, vals, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -63,6 +63,9 @@ struct ReactArgs {
   int use_pred;                          // 1: the learned predictor supplies each Newton attempt's
                                          // initial iterate (compacted schedule only)
   unsigned int* counts;                  // 8 counters, see run_react
+  // optional cost ordering (mi.order): workspace slot s integrates the caller's cell
+  // slot_cell[s]; null = slot s is cell s
+  const int* slot_cell;
   // outputs
   unsigned long long* counters;          // per-call aggregate counters (see CounterIndex)
   int* cell_status;                      // optional per-cell status (nullable)
@@ -131,6 +134,18 @@ int persist_capacity(int mech, int occupancy);
 // iteration count of a wave and a spare
 hipError_t run_react_persist(int mech, const ReactArgs& a, const PersistPlan& plan, long max_trips,
                              hipStream_t stream);
+// cost-ordering keys of every caller cell (ORDER_FC: the |FC| the caller's array holds;
+// ORDER_RATE: the fastest species' change over dt in tolerance units at the input state)
+// and the identity as values, for the sort that gives ReactArgs::slot_cell
+enum OrderMode : int { ORDER_NONE = 0, ORDER_RATE = 1, ORDER_FC = 2 };
+hipError_t launch_cost_keys(int mech, const ReactArgs& a, int mode, unsigned int* keys, int* vals,
+                            hipStream_t stream);
+// stable ascending radix sort of (keys, vals) pairs with hipCUB (cost_order.hip); the
+// buffers hold 2 n entries each (double buffering); *sorted_vals receives the half that
+// holds the result
+size_t cost_sort_temp_bytes(long n);
+hipError_t cost_sort(void* temp, size_t temp_bytes, unsigned int* keys, int* vals, long n, hipStream_t stream,
+                     const int** sorted_vals);
 hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t stream);
 // reactionRateRhoY over a box or a flat array of cells (rates_kernel)
 struct RatesArgs {

@@ -10,6 +10,17 @@ thread_local Idx3 threadIdx, blockIdx, blockDim, gridDim;
 using namespace pelelm;
 
 static int g_lu_mode = LU_FP64, g_ir_sweeps = 3, g_pred = 0, g_fill = -1;
+static const int* g_order = nullptr;
+static const double *g_fy = nullptr, *g_fh = nullptr;
+static const int* g_mask = nullptr;
+// forcing (rYsrc [nspec][ncell], rEsrc [ncell]) and mask of the following emu_react calls
+extern "C" void emu_set_forcing(const double* fy, const double* fh, const int* mask) {
+  g_fy = fy;
+  g_fh = fh;
+  g_mask = mask;
+}
+// slot -> cell map of the following emu_react calls (the cost ordering; null: identity)
+extern "C" void emu_set_order(const int* slot_cell) { g_order = slot_cell; }
 // byte pattern the workspace, lists and matrix region are filled with before a run
 // (-1: left as allocated), so a read of a word the integrator never wrote shows up
 extern "C" void emu_set_fill(int byte) { g_fill = byte; }
@@ -63,6 +74,10 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   a.ir_sweeps = g_ir_sweeps;
   a.ir_tol = 1e-4;
   a.counters = counters;
+  a.slot_cell = g_order;
+  if (g_fy) { a.rYsrc.p = const_cast<double*>(g_fy); a.rYsrc.ns = ncell; }
+  if (g_fh) { a.rEsrc.p = const_cast<double*>(g_fh); a.rEsrc.ns = ncell; }
+  a.mask.p = g_mask;
   MlpDev md{};
   std::vector<unsigned short> wz(2 * 64 * 128 * 8, 0);
   std::vector<float> bz(4 * 128, 0.f), norm(256, 0.f);
@@ -90,5 +105,29 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
 extern "C" int emu_rates(const char* mech, long ncell, const double* rY, const double* rH, double* T, double* out) {
   const int m = mech_lookup(mech);
   if (m < 0) return -1;
-  return (int)launch_rates(m, rY, rH, T, nullptr, out, ncell, nullptr);
+  RatesArgs ra;
+  ra.rhoY.p = const_cast<double*>(rY); ra.rhoY.ns = ncell;
+  ra.rhoH.p = const_cast<double*>(rH); ra.rhoH.ns = ncell;
+  ra.T.p = T; ra.T.ns = ncell;
+  ra.rdot.p = out; ra.rdot.ns = ncell;
+  ra.Tout = T;
+  ra.nx = (int)ncell;
+  ra.ncell = ncell;
+  return (int)launch_rates(m, ra, nullptr);
+}
+
+// the cost-ordering keys (mi.order) of flat cells with scalar tolerances
+extern "C" int emu_cost_keys(const char* mech, long ncell, double* rY, double* T, double* rE, double* FC,
+                             double dt, double rtol, double atol, int mode, unsigned int* keys, int* vals) {
+  const int m = mech_lookup(mech);
+  if (m < 0) return -1;
+  ReactArgs a{};
+  a.rY.p = rY; a.rY.ns = ncell;
+  a.T.p = T; a.T.ns = ncell;
+  a.rE.p = rE; a.rE.ns = ncell;
+  a.FC.p = FC; a.FC.ns = ncell;
+  a.nx = (int)ncell; a.ny = 1; a.ncell = ncell;
+  a.dt = dt; a.rtol = rtol;
+  for (int i = 0; i <= mech_nspec(m); ++i) a.atol[i] = atol;
+  return (int)launch_cost_keys(m, a, mode, keys, vals, nullptr);
 }

@@ -75,6 +75,11 @@ inline long long __double_as_longlong(double x) { long long r; __builtin_memcpy(
 inline double __longlong_as_double(long long x) { double r; __builtin_memcpy(&r, &x, 8); return r; }
 #define __builtin_amdgcn_readlane(v, l) (v)
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline unsigned int __float_as_uint(float x) {
+  unsigned int u;
+  __builtin_memcpy(&u, &x, 4);
+  return u;
+}
 template <class T> inline T atomicAdd(T* p, T v) { T o = *p; *p += v; return o; }
 template <class T> inline T atomicMax(T* p, T v) { T o = *p; if (v > o) *p = v; return o; }
 

@@ -327,6 +327,11 @@ inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline long long __double_as_longlong(double x) { long long r; std::memcpy(&r, &x, 8); return r; }
 inline double __longlong_as_double(long long x) { double r; std::memcpy(&r, &x, 8); return r; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline unsigned int __float_as_uint(float x) {
+  unsigned int u;
+  __builtin_memcpy(&u, &x, 4);
+  return u;
+}
 template <class T> inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
 template <class T> inline T atomicMax(T* p, T v) {
   T o = __atomic_load_n(p, __ATOMIC_RELAXED);

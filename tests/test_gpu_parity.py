@@ -396,3 +396,40 @@ def test_one_lane_solves_in_graph_trips_cover_every_cell():
     r.close()
     assert st["nfail"] == 0 and st["graph_trips_used"] == 8
     _parity("lidryer 32768 lane graphs", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("schedule,graphs", [("plain", 0), ("queue", 0), ("plain", 8)])
+def test_cost_ordering_changes_nothing_but_the_lanes(schedule, graphs):
+    """mi.order = rate and mi.order = fc (the RHS counts of the previous call, which the FC
+    array holds on entry as PeleLM's FuncCount does) put cells of similar cost in the same
+    waves; every cell's result and RHS count stay bitwise those of the unordered call, with
+    masked cells and forcing, and match the oracle."""
+    n = 8192
+    d = cells.generate_workload("drm19_128_mix", ncell=n)
+    rng = np.random.default_rng(21)
+    mask = np.where(rng.random(n) < 0.1, -1, 1).astype(np.int32)
+    fY = rng.normal(size=d["rY"].shape) * 1e-3 * d["rY"] / d["dt"]
+    fH = rng.normal(size=n) * 1e-3 * np.abs(d["rE"]) / d["dt"]
+    ref = O.Mech("drm19").react(d["rY"], d["T"], d["rE"], d["dt"], rYsrc=fY, rEsrc=fH, mask=mask,
+                                rtol=1e-6, atol=1e-10)
+    out = {}
+    prev_fc = None
+    for order in ("none", "rate", "fc"):
+        r = _mk("drm19", ode__rtol=1e-6, ode__atol=1e-10, mi__schedule=schedule, mi__graph_trips=graphs,
+                mi__order=order)
+        rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+        FC = _dev(prev_fc if order == "fc" else np.zeros(n))
+        r.react(rY, _dev(fY), T, rE, _dev(fH), FC, _dev(mask, torch.int32), d["dt"], 0.0)
+        torch.cuda.synchronize()
+        st = r.stats()
+        r.close()
+        assert st["nfail"] == 0
+        out[order] = [x.cpu().numpy() for x in (rY, T, rE, FC)]
+        print(f"{schedule} graphs {graphs} order {order}: {st['ms_kernel']:.2f} ms, {st['ntrips']} trips")
+        if order == "none":
+            prev_fc = out[order][3]
+    for order in ("rate", "fc"):
+        for a, b in zip(out["none"], out[order]):
+            np.testing.assert_array_equal(a, b)
+    gY, gT, _, gFC = out["none"]
+    _parity(f"ordered {schedule}", gY, gT, gFC, ref)

@@ -120,3 +120,56 @@ def test_emulated_fp32_factors_take_the_fp64_steps(emu, wl, mech, n):
     tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
     assert np.max(np.abs(rY - ref["rY"]) / tol) < 0.05
     assert np.max(np.abs(T - ref["T"])) < 1e-6
+
+
+@pytest.mark.parametrize("compact", [1, 0])
+def test_emulated_cost_ordering_changes_nothing_but_the_slots(emu, compact):
+    """With a slot -> cell map (mi.order), every cell's result and RHS count are those of
+    the unordered run: only the workspace slot a cell occupies changes."""
+    n = 192
+    d = cells.generate_workload("drm19_128_mix", ncell=n)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    out = []
+    perm = np.random.default_rng(3).permutation(n).astype(np.int32)
+    for order in (None, perm):
+        rY, T, rE = d["rY"].copy(), d["T"].copy(), d["rE"].copy()
+        FC = np.zeros(n)
+        cnt = np.zeros(16, dtype=np.uint64)
+        trips = C.c_long(0)
+        emu.emu_set_order(None if order is None else p(order))
+        try:
+            rc = emu.emu_react(b"drm19", C.c_long(n), p(rY), p(T), p(rE), p(FC), C.c_double(d["dt"]),
+                               C.c_double(1e-6), C.c_double(1e-10), 4, compact, p(cnt), C.byref(trips))
+        finally:
+            emu.emu_set_order(None)
+        assert rc == 0
+        out.append((rY, T, rE, FC))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_emulated_cost_keys(emu):
+    """The ordering keys: ORDER_RATE is max_k |W_k wdot_k| dt / (rtol rhoY_k + atol) at the
+    input state (T from HY2T), computed here from the oracle's rates; ORDER_FC is |FC|."""
+    n = 64
+    d = cells.generate_workload("drm19_128_mix", ncell=n)
+    m = O.Mech("drm19")
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    rY, T, rE = d["rY"].copy(), d["T"].copy() + 5.0, d["rE"].copy()
+    FC = -np.arange(n, dtype=np.float64)
+    keys = np.zeros(n, dtype=np.uint32)
+    vals = np.zeros(n, dtype=np.int32)
+    rtol, atol = 1e-6, 1e-10
+    assert emu.emu_cost_keys(b"drm19", C.c_long(n), p(rY), p(T), p(rE), p(FC), C.c_double(d["dt"]),
+                             C.c_double(rtol), C.c_double(atol), 1, p(keys), p(vals)) == 0
+    got = keys.view(np.float32)
+    np.testing.assert_array_equal(vals, np.arange(n))
+    for c in range(n):
+        rho = rY[:, c].sum()
+        Tc, _ = m.hy2t(rE[c] / rho, rY[:, c] / rho, T[c])
+        w = m.wdot(rY[:, c] / m.W, Tc) * m.W
+        ref = np.max(np.abs(w) * d["dt"] / (rtol * np.abs(rY[:, c]) + atol))
+        assert abs(got[c] - ref) <= 1e-5 * ref
+    assert emu.emu_cost_keys(b"drm19", C.c_long(n), p(rY), p(T), p(rE), p(FC), C.c_double(d["dt"]),
+                             C.c_double(rtol), C.c_double(atol), 2, p(keys), p(vals)) == 0
+    np.testing.assert_array_equal(keys.view(np.float32), np.arange(n, dtype=np.float32))
