@@ -131,6 +131,23 @@ int mi_chem_react_flat(void* reactor, double* rY, const double* rYsrc, double* T
                        const double* rEsrc, double* FC, const int* mask, int ncells, double* dt, double* time,
                        void* hip_stream);
 
+/* mi_chem_react_flat that allocates nothing.  No counterpart in the reference: it is for
+   callers that cannot allocate device memory during the call (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  The cells integrate in
+   the persistent schedule (one kernel; lanes claim cells from a device counter), whose
+   lane slots, queue counters and statistics counters live in a file-scope device array of
+   256 MiB per GPU, held until the process ends: at most as many waves of 64 lane slots as
+   fit there (drm19 about 250, the 53-species mechanism about 50), however many cells the
+   call has.  A mechanism whose one wave of slots would not fit is rejected with
+   MI_ENOMEM.  The reactor's settings (tolerances, mechanism, mi.lu, mi.persist_chunk,
+   mi.on_fail) apply; mi.schedule and mi.order do not.  Every call on a GPU shares that
+   GPU's array: calls that overlap on one GPU race, and mi_chem_get_stats reports the last
+   call on the array.  mi_chem_init must have run on the device (it records the array's
+   address and the kernel's occupancy). */
+int mi_chem_react_flat_noalloc(void* reactor, double* rY, const double* rYsrc, double* T, double* rE,
+                               const double* rEsrc, double* FC, const int* mask, int ncells, double* dt,
+                               double* time, void* hip_stream);
+
 /* Net production rates rho*wdot (CGS, g/cm^3/s) of flat cells from (rhoY, rhoH, T) --
    reactionRateRhoY (Source/PeleLM_K.H:57-98) in CGS units: T is re-solved from
    h = rhoH/rho by HY2T starting from T, which is read only; the re-solved temperatures

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -81,6 +82,9 @@ struct Reactor {
   long glue_cells = 0;
   unsigned int* counts = nullptr;
   unsigned long long* counters = nullptr;
+  // where the last call's counters are (the reactor's, or the no-allocation array's)
+  unsigned int* stat_counts = nullptr;
+  unsigned long long* stat_counters = nullptr;
   long last_trips = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipStream_t last_stream = nullptr;
@@ -119,8 +123,8 @@ int ensure_device_state(Reactor* r, long ncell) {
     HIPCHK(r, hipGetDevice(&r->device));
     HIPCHK(r, hipMalloc(&r->counts, 8 * sizeof(unsigned int)));
     HIPCHK(r, hipMalloc(&r->counters, NCOUNTERS * sizeof(unsigned long long)));
-    HIPCHK(r, hipEventCreate(&r->ev0));
-    HIPCHK(r, hipEventCreate(&r->ev1));
+    if (!r->ev0) HIPCHK(r, hipEventCreate(&r->ev0));
+    if (!r->ev1) HIPCHK(r, hipEventCreate(&r->ev1));
   }
   if (ncell > r->ws_cells) {
     free_workspace(r);
@@ -257,16 +261,29 @@ int autotune(Reactor* r, const ReactArgs& a0, hipStream_t stream, long max_trips
 int failure_code(Reactor* r, hipStream_t stream) {
   if (r->on_fail_continue || r->null_reactor) return MI_OK;
   unsigned long long nfail = 0;
-  HIPCHK(r, hipMemcpyAsync(&nfail, r->counters + CNT_NFAIL, sizeof(nfail), hipMemcpyDeviceToHost, stream));
+  HIPCHK(r, hipMemcpyAsync(&nfail, r->stat_counters + CNT_NFAIL, sizeof(nfail), hipMemcpyDeviceToHost, stream));
   HIPCHK(r, hipStreamSynchronize(stream));
   if (nfail > 0)
     return fail(r, MI_ECELLFAIL, std::to_string(nfail) + " cell(s) failed to integrate (FC < 0 marks them)");
   return MI_OK;
 }
 
+// Device memory placed by the caller of react_persist instead of the reactor's workspace
+// (mi_chem_react_flat_noalloc): lane slots for at most max_waves waves and the counters.
+struct PersistMem {
+  double* ws;
+  int* iws;
+  double* mat;
+  unsigned int* counts;
+  unsigned long long* counters;
+  long max_waves;
+};
+
 // mi.schedule = persistent: the workspace holds the kernel's lane slots (whatever the box
-// size); the kernels are enqueued on `stream` with no host synchronisation
-int react_persist(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream) {
+// size); the kernels are enqueued on `stream` with no host synchronisation.  With `fixed`
+// the slots and counters are that memory and nothing is allocated.
+int react_persist(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream,
+                  const PersistMem* fixed = nullptr) {
   PersistPlan plan;
   plan.occupancy = r->persist_occ;
   plan.chunk = r->persist_chunk;
@@ -280,23 +297,35 @@ int react_persist(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_
   const bool capturing = cs != hipStreamCaptureStatusNone;
   long waves = r->persist_waves > 0 ? r->persist_waves : cap;
   waves = std::min(waves, (a.ncell + 63) / 64);
+  if (fixed) waves = std::min(waves, fixed->max_waves);
   plan.waves = (int)std::max(1L, waves);
-  int rc = ensure_device_state(r, (long)plan.waves * 64);
-  if (rc) return rc;
-  a.ws = r->ws;
-  a.iws = r->iws;
-  a.mat = r->mat;
-  a.counts = r->counts;
-  a.counters = r->counters;
+  if (fixed) {
+    a.ws = fixed->ws;
+    a.iws = fixed->iws;
+    a.mat = fixed->mat;
+    a.counts = fixed->counts;
+    a.counters = fixed->counters;
+  } else {
+    int rc = ensure_device_state(r, (long)plan.waves * 64);
+    if (rc) return rc;
+    a.ws = r->ws;
+    a.iws = r->iws;
+    a.mat = r->mat;
+    a.counts = r->counts;
+    a.counters = r->counters;
+  }
+  r->stat_counts = a.counts;
+  r->stat_counters = a.counters;
   a.compact = 0;
   a.use_pred = 0;
   const long max_trips = r->max_trips > 0 ? r->max_trips : 3 * r->max_steps + 100;
   r->last_schedule = SCHED_PERSIST;
   r->last_graph = 0;
-  HIPCHK(r, launch_counters_reset(r->counters, stream));
-  if (!capturing) HIPCHK(r, hipEventRecord(r->ev0, stream));
+  const bool timed = !capturing && r->ev0 && r->ev1;
+  HIPCHK(r, launch_counters_reset(a.counters, stream));
+  if (timed) HIPCHK(r, hipEventRecord(r->ev0, stream));
   HIPCHK(r, run_react_persist(r->mech, a, plan, max_trips, stream));
-  if (!capturing) HIPCHK(r, hipEventRecord(r->ev1, stream));
+  if (timed) HIPCHK(r, hipEventRecord(r->ev1, stream));
   r->prof = KernelProf{};
   r->last_stream = stream;
   r->have_stats = true;
@@ -305,7 +334,8 @@ int react_persist(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_
   return capturing ? MI_OK : failure_code(r, stream);
 }
 
-int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream) {
+int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream,
+                 const PersistMem* fixed = nullptr) {
   if (!r->inited) return fail(r, MI_ESTATE, "react called before init");
   if (!dt || !(*dt > 0.0)) return fail(r, MI_EINVAL, "dt must be positive");
   if (!a.rY.p || !a.T.p || !a.rE.p || !a.FC.p) return fail(r, MI_EINVAL, "rY, T, rE and FC are required");
@@ -329,6 +359,15 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
     *time += *dt;
     return MI_OK;
   }
+  if (fixed) {   // the no-allocation entry point: the persistent schedule in fixed memory
+    if (r->null_reactor) {
+      HIPCHK(r, launch_null(N, a, stream));
+      r->have_stats = false;
+      *time += *dt;
+      return MI_OK;
+    }
+    return react_persist(r, a, dt, time, stream, fixed);
+  }
   if (r->schedule == SCHED_PERSIST && !r->null_reactor) return react_persist(r, a, dt, time, stream);
   int rc = ensure_device_state(r, r->null_reactor ? 0 : a.ncell);
   if (rc) return rc;
@@ -344,6 +383,8 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   a.use_pred = (r->predictor && !r->null_reactor) ? 1 : 0;
   a.counts = r->counts;
   a.counters = r->counters;
+  r->stat_counts = r->counts;
+  r->stat_counters = r->counters;
   // every step takes at least one trip; 3 per step bounds retries generously
   const long max_trips = r->max_trips > 0 ? r->max_trips : 3 * r->max_steps + 100;
   if (graph_trips > 0 || r->schedule == SCHED_AUTO) {
@@ -432,6 +473,69 @@ bool box_covers(const V& x, const int lo[3], const int hi[3], int ncomp) {
   return x.ncomp >= ncomp;
 }
 
+// ---------------------------------------------------------------------------
+// Device memory of mi_chem_react_flat_noalloc: the persistent schedule's lane slots
+// (workspace tiles and matrix regions) and the call's queue and statistics counters,
+// carved from one file-scope array.  Every GPU the library runs on holds its own copy
+// until the process ends.  Host code never uses the array's name as an address: the
+// address of the current device's copy comes from hipGetSymbolAddress, once per device.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_BYTES = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mutex;
+unsigned char* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the scratch array's address on the current device
+int noalloc_scratch(Reactor* r, unsigned char** out) {
+  int dev = 0;
+  HIPCHK(r, hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(r, MI_EHIP, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(r, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return MI_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// the scratch array's layout for `waves` waves of lane slots of mechanism `mech`; returns
+// the bytes used
+size_t noalloc_layout(int mech, long waves, unsigned char* base, PersistMem* m) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base ? base + off : nullptr;
+    off = align256(off + bytes);
+    return p;
+  };
+  unsigned char* counts = take(8 * sizeof(unsigned int));
+  unsigned char* counters = take(NCOUNTERS * sizeof(unsigned long long));
+  unsigned char* ws = take((size_t)waves * ws_slots(mech) * 64 * sizeof(double));
+  unsigned char* iws = take((size_t)waves * iws_slots(mech) * 64 * sizeof(int));
+  unsigned char* mat = take((size_t)waves * 64 * mat_stride(mech) * sizeof(double));
+  if (m) {
+    m->counts = reinterpret_cast<unsigned int*>(counts);
+    m->counters = reinterpret_cast<unsigned long long*>(counters);
+    m->ws = reinterpret_cast<double*>(ws);
+    m->iws = reinterpret_cast<int*>(iws);
+    m->mat = reinterpret_cast<double*>(mat);
+    m->max_waves = waves;
+  }
+  return off;
+}
+
+// the most waves of lane slots of `mech` the scratch array holds
+long noalloc_max_waves(int mech) {
+  const size_t fixed = noalloc_layout(mech, 0, nullptr, nullptr);
+  const size_t per_wave = noalloc_layout(mech, 1, nullptr, nullptr) - fixed + 3 * 256;
+  return NOALLOC_BYTES > fixed ? (long)((NOALLOC_BYTES - fixed) / per_wave) : 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -460,6 +564,24 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells) {
   if (reactor_type != 2) return fail(r, MI_EINVAL, "only reactor_type 2 (h-reactor, constant pressure) is supported");
   if (ncells < 0) return fail(r, MI_EINVAL, "ncells must be >= 0");
   r->reactor_type = reactor_type;
+  // set-up for the no-allocation entry point, when a device is present: the call's
+  // timing events, the scratch array's address on this device and the persistent
+  // kernel's resident waves (without them that entry point takes the address and the
+  // occupancy on its first call and records no timing)
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) {
+    if (!r->ev0) HIPCHK(r, hipEventCreate(&r->ev0));
+    if (!r->ev1) HIPCHK(r, hipEventCreate(&r->ev1));
+    unsigned char* base = nullptr;
+    const int rc = noalloc_scratch(r, &base);
+    if (rc) return rc;
+    if (!r->null_reactor) {
+      int& cap = r->persist_cap[r->mech][r->persist_occ - 1];
+      if (cap <= 0) cap = persist_capacity(r->mech, r->persist_occ);
+    }
+  } else {
+    (void)hipGetLastError();
+  }
   r->inited = true;
   return MI_OK;
 }
@@ -579,6 +701,37 @@ int mi_chem_react_flat(void* reactor, double* rY, const double* rYsrc, double* T
   a.ny = 1;
   a.ncell = ncells;
   return react_common(r, a, dt, time, static_cast<hipStream_t>(hip_stream));
+}
+
+int mi_chem_react_flat_noalloc(void* reactor, double* rY, const double* rYsrc, double* T, double* rE,
+                               const double* rEsrc, double* FC, const int* mask, int ncells, double* dt,
+                               double* time, void* hip_stream) {
+  auto* r = static_cast<Reactor*>(reactor);
+  if (!r) return MI_EINVAL;
+  if (ncells < 0) return fail(r, MI_EINVAL, "ncells must be >= 0");
+  if (!time) return fail(r, MI_EINVAL, "time is required");
+  if (!r->inited) return fail(r, MI_ESTATE, "react called before init");
+  const long maxw = noalloc_max_waves(r->mech);
+  if (maxw < 1)
+    return fail(r, MI_ENOMEM, "one wave of lane slots of this mechanism needs more than the " +
+                                  std::to_string(NOALLOC_BYTES >> 20) + " MiB no-allocation array");
+  unsigned char* base = nullptr;
+  const int rc = noalloc_scratch(r, &base);
+  if (rc) return rc;
+  PersistMem m{};
+  noalloc_layout(r->mech, maxw, base, &m);
+  ReactArgs a{};
+  a.rY = flat_view(rY, ncells);
+  a.rYsrc = flat_view(const_cast<double*>(rYsrc), ncells);
+  a.T = flat_view(T, ncells);
+  a.rE = flat_view(rE, ncells);
+  a.rEsrc = flat_view(const_cast<double*>(rEsrc), ncells);
+  a.FC = flat_view(FC, ncells);
+  a.mask.p = mask;
+  a.nx = std::max(ncells, 1);
+  a.ny = 1;
+  a.ncell = ncells;
+  return react_common(r, a, dt, time, static_cast<hipStream_t>(hip_stream), &m);
 }
 
 int mi_chem_react_box(void* reactor, const int lo[3], const int hi[3], mi_real4 rY, mi_real4 rYsrc, mi_real4 T,
@@ -744,9 +897,10 @@ int mi_chem_get_stats(void* reactor, mi_chem_stats* out) {
   if (!r || !out) return MI_EINVAL;
   std::memset(out, 0, sizeof(*out));
   if (!r->have_stats) return MI_OK;
-  HIPCHK(r, hipEventSynchronize(r->ev1));
+  if (r->ev1) HIPCHK(r, hipEventSynchronize(r->ev1));
+  else HIPCHK(r, hipStreamSynchronize(r->last_stream));
   unsigned long long c[NCOUNTERS];
-  HIPCHK(r, hipMemcpy(c, r->counters, sizeof(c), hipMemcpyDeviceToHost));
+  HIPCHK(r, hipMemcpy(c, r->stat_counters, sizeof(c), hipMemcpyDeviceToHost));
   out->ncells = (long)c[CNT_CELLS];
   out->nsteps = (long)c[CNT_NST];
   out->nfe = (long)c[CNT_NFE];
@@ -763,12 +917,12 @@ int mi_chem_get_stats(void* reactor, mi_chem_stats* out) {
   out->npred_used = (long)c[CNT_PRED];
   out->npred_fallback = (long)c[CNT_PREDFB];
   float ms = 0.f;
-  HIPCHK(r, hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  if (r->ev0 && r->ev1) HIPCHK(r, hipEventElapsedTime(&ms, r->ev0, r->ev1));
   out->ms_kernel = ms;
   out->ntrips = r->last_trips;
   if (r->last_schedule == SCHED_PERSIST) {   // the most iterations (trips) of a wave
     unsigned int it = 0;
-    HIPCHK(r, hipMemcpy(&it, r->counts + 2, sizeof(it), hipMemcpyDeviceToHost));
+    HIPCHK(r, hipMemcpy(&it, r->stat_counts + 2, sizeof(it), hipMemcpyDeviceToHost));
     out->ntrips = it;
   }
   for (int kk = 0; kk < MI_NKERNEL_KINDS && kk < NKK; ++kk) {

@@ -46,7 +46,7 @@ class MiStats(C.Structure):
 
 # Every exported entry point of include/pelelm_chem.h.
 EXPORTS = ["mi_chem_create", "mi_chem_init", "mi_chem_set", "mi_chem_set_typ_vals", "mi_chem_react_box",
-           "mi_chem_react_flat", "mi_chem_reaction_rates_flat", "mi_chem_reaction_rates_box", "mi_chem_advance_box",
+           "mi_chem_react_flat", "mi_chem_react_flat_noalloc", "mi_chem_reaction_rates_flat", "mi_chem_reaction_rates_box", "mi_chem_advance_box",
            "mi_chem_rhoh_to_temp_box",
            "mi_chem_get_stats", "mi_chem_nspec", "mi_chem_last_error", "mi_chem_close"]
 
@@ -69,6 +69,7 @@ def lib():
                                         MiReal4, MiInt4, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
         L.mi_chem_react_flat.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double), vp]
+        L.mi_chem_react_flat_noalloc.argtypes = L.mi_chem_react_flat.argtypes
         L.mi_chem_reaction_rates_flat.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
         L.mi_chem_reaction_rates_box.argtypes = [vp, C.c_int * 3, C.c_int * 3, MiReal4, MiReal4, MiReal4, MiInt4,
                                                  MiReal4, vp]
@@ -162,9 +163,12 @@ class ReactorBase:
             pass
 
     # -- work --------------------------------------------------------------
-    def react(self, rY, rYsrc, T, rE, rEsrc, FC, mask, dt: float, time: float = 0.0, stream=None):
+    def react(self, rY, rYsrc, T, rE, rEsrc, FC, mask, dt: float, time: float = 0.0, stream=None,
+              noalloc: bool = False):
         """Flat SoA form: rY [nspec, ncell], rYsrc [nspec, ncell] or None, T/rE/FC [ncell],
-        rEsrc [ncell] or None, mask [ncell] int32 or None.  Returns the updated (dt, time)."""
+        rEsrc [ncell] or None, mask [ncell] int32 or None.  Returns the updated (dt, time).
+        noalloc=True calls mi_chem_react_flat_noalloc (persistent schedule in the library's
+        fixed device array; nothing is allocated during the call)."""
         ncell = T.shape[0]
         for name, t, n in (("rY", rY, self.nspec), ("rYsrc", rYsrc, self.nspec)):
             if t is not None and tuple(t.shape) != (n, ncell):
@@ -173,9 +177,9 @@ class ReactorBase:
             if t is not None and tuple(t.shape) != (ncell,):
                 raise ValueError(f"{name} must have shape ({ncell},)")
         d, tm = C.c_double(dt), C.c_double(time)
-        self._check(lib().mi_chem_react_flat(self._h, _ptr(rY), _ptr(rYsrc), _ptr(T), _ptr(rE), _ptr(rEsrc),
-                                             _ptr(FC), _ptr(mask), ncell, C.byref(d), C.byref(tm),
-                                             _stream(stream)))
+        fn = lib().mi_chem_react_flat_noalloc if noalloc else lib().mi_chem_react_flat
+        self._check(fn(self._h, _ptr(rY), _ptr(rYsrc), _ptr(T), _ptr(rE), _ptr(rEsrc), _ptr(FC), _ptr(mask), ncell,
+                       C.byref(d), C.byref(tm), _stream(stream)))
         return d.value, tm.value
 
     def react_box(self, lo, hi, rY: MiReal4, rYsrc: MiReal4, T: MiReal4, rE: MiReal4, rEsrc: MiReal4,
