@@ -98,6 +98,9 @@ int orc_react(const orc_mech* m, int ncell, double* rY, const double* rYsrc, dou
 int orc_radau_react(const orc_mech* m, int ncell, double* rY, const double* rYsrc, double* T, double* rE,
                     const double* rEsrc, double dt, double rtol, double atol, int nthreads, long* nsteps);
 
+/* Radau IIA-5 coefficients as radau5.cpp derives them: c[3], A[3][3], e[3], g0. */
+void orc_radau_coefficients(double* out16);
+
 /* Integrate the linear test system y' = A y (n x n, row-major) with the same BDF
    machinery (pins the integrator against exp(A t)).  Returns status. */
 int orc_bdf_linear(int n, const double* A, double* y, double tout, const orc_opts* opts, orc_cellstats* st);

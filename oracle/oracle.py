@@ -76,6 +76,7 @@ def lib():
                                           C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int,
                                           C.c_void_p]
             L.orc_radau_react.restype = C.c_int
+            L.orc_radau_coefficients.argtypes = [_dp]
         _lib = L
     return _lib
 
@@ -190,6 +191,13 @@ class Mech:
                                       None if esrc is None else esrc.ctypes.data, float(dt), float(rtol),
                                       float(atol), int(nthreads), None)
         return dict(rY=rY, T=T, rE=rE, nfail=nfail)
+
+
+def radau_coefficients():
+    """(c, A, e, g0) of the Radau IIA-5 integrator as the oracle derived them."""
+    out = np.zeros(16)
+    lib().orc_radau_coefficients(out)
+    return out[:3], out[3:12].reshape(3, 3), out[12:15], out[15]
 
 
 def bdf_linear(A, y0, tout, rtol=1e-8, atol=1e-12, max_order=5):

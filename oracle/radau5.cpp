@@ -326,6 +326,17 @@ struct Radau {
 
 }  // namespace
 
+// the method's coefficients as start-up derived them: c (3), A (3 x 3 row-major), the
+// error weights e (3) and g0, 16 values
+extern "C" void orc_radau_coefficients(double* out) {
+  const RadauCoef& rc = coef();
+  for (int i = 0; i < 3; ++i) out[i] = rc.c[i];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[3 + 3 * i + j] = rc.A[i][j];
+  for (int i = 0; i < 3; ++i) out[12 + i] = rc.e[i];
+  out[15] = rc.g0;
+}
+
 extern "C" int orc_radau_react(const orc_mech* m, int ncell, double* rY, const double* rYsrc, double* T,
                                double* rE, const double* rEsrc, double dt, double rtol, double atol,
                                int nthreads, long* nsteps) {

@@ -160,6 +160,24 @@ def test_bdf_matches_radau_fixture(wl):
     assert np.max(np.abs(loose["T"] - ref[N]) / ref[N]) < 1e-5
 
 
+def test_radau_coefficients_are_the_published_method():
+    """The coefficients radau5.cpp derives at start-up against the closed forms of Radau
+    IIA-5 (Hairer & Wanner II, table IV.5.6) and the RADAU5 error estimate: A's first row
+    ((88 - 7 sqrt6)/360, (296 - 169 sqrt6)/1800, (-2 + 3 sqrt6)/225), b = A's last row
+    ((16 - sqrt6)/36, (16 + sqrt6)/36, 1/9), g0 = 1 / (real eigenvalue of A^-1), and
+    e = g0 (-(13 + 7 sqrt6)/3, (-13 + 7 sqrt6)/3, -1/3)."""
+    c, A, e, g0 = O.radau_coefficients()
+    s6 = np.sqrt(6.0)
+    np.testing.assert_allclose(c, [(4 - s6) / 10, (4 + s6) / 10, 1.0], rtol=1e-15)
+    np.testing.assert_allclose(A[0], [(88 - 7 * s6) / 360, (296 - 169 * s6) / 1800, (-2 + 3 * s6) / 225], rtol=1e-13)
+    np.testing.assert_allclose(A[2], [(16 - s6) / 36, (16 + s6) / 36, 1 / 9], rtol=1e-13)
+    lam = np.linalg.eigvals(np.linalg.inv(A))
+    real = lam[np.abs(lam.imag) < 1e-9].real
+    assert len(real) == 1
+    np.testing.assert_allclose(g0, 1.0 / real[0], rtol=1e-13)
+    np.testing.assert_allclose(e, g0 * np.array([-(13 + 7 * s6) / 3, (-13 + 7 * s6) / 3, -1 / 3]), rtol=1e-12)
+
+
 @pytest.mark.parametrize("wl", ["lidryer_16", "drm19_64", "drm19_128_mix", "dodecane53_64_mix"])
 def test_radau_matches_scipy_radau_fixture(wl):
     """The C++ Radau IIA-5 (oracle/radau5.cpp) against SciPy's Radau on the same cells:
