@@ -6,9 +6,12 @@
  * `peleLM.chem_integrator` name, initialised as an h-reactor, given typical values for
  * the absolute tolerances, asked to react boxes, closed at teardown.  Every entry point
  * takes plain pointers and sizes; device arrays are HIP device pointers and all work is
- * enqueued on the caller's HIP stream.  The host-driven schedules read the work-list
- * counts back once per trip, so react returns with the integration finished (the
- * reference's caller synchronises right after the call anyway, Source/PeleLM.cpp:5773-5775).
+ * enqueued on the caller's HIP stream.  The host-driven schedules (plain, queue, auto)
+ * read the work-list counts back once per trip, so react returns with the integration
+ * finished (the reference's caller synchronises right after the call anyway,
+ * Source/PeleLM.cpp:5773-5775); the persistent schedule enqueues one kernel and
+ * synchronises only to read the failure count under mi.on_fail = abort (never under
+ * stream capture).
  *
  * Units are CGS in and out, as the reference converts before and after the call
  * (Source/PeleLM.cpp:5737-5746, 5764-5771).
@@ -80,7 +83,13 @@ typedef struct {
 
 /* ReactorBase::create(name) -- Source/PeleLM.cpp:706.
    Accepted names: "ReactorCvode" / "ReactorBDF" (this reactor; dense analytic-Jacobian
-   BDF on the GPU) and "ReactorNull" (pass-through: rY += dt rYsrc, rE += dt rEsrc). */
+   BDF on the GPU), "ReactorRK64" (explicit six-stage Runge-Kutta sub-steps for cells that
+   are not stiff, the reactor Tools/CMake/BuildPeleLMExe.cmake:124-125 builds: fourth-order
+   solution with a fifth-order error estimate in the rtol/atol norm, sub-steps within
+   [dt / ode.rk64_nsubsteps_max, dt / ode.rk64_nsubsteps_min] from
+   dt / ode.rk64_nsubsteps_guess (defaults 500, 5, 10); a cell that fails the error test at
+   the smallest sub-step fails) and "ReactorNull" (pass-through: rY += dt rYsrc,
+   rE += dt rEsrc). */
 int mi_chem_create(const char* integrator, void** reactor);
 
 /* ReactorBase::init(reactor_type, ncells) -- Source/PeleLM.cpp:707-709.
@@ -95,7 +104,15 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
      mi.mechanism (drm19 | LiDryer | dodecane53), mi.schedule (plain: fixed cell-to-lane mapping
      for the evaluation and step kernels, default | queue: finished cells leave the work
      lists | auto: the first call of each box size times both, with and without graph
-     capture, on a copy of its cells and keeps the fastest), mi.graph_trips (0: the host
+     capture, on a copy of its cells and keeps the fastest | persistent: one kernel per
+     call, waves keep their cells through all trips and claim new ones from a device
+     counter), mi.persist_waves (0: as many as the device holds resident), mi.persist_chunk
+     (8: idle lanes of a wave that trigger a claim, 1..64), mi.persist_occupancy (1 | 2
+     waves per SIMD), mi.order (rate: cells sorted by their initial rates' change over dt
+     in tolerance units, so a wave's cells finish together, default | fc: sorted by the
+     |FC| the FC array holds on entry, the previous step's counts when the caller passes
+     them | none; plain, queue and auto schedules; results do not depend on it),
+     mi.graph_trips (0: the host
      issues every trip, default | K even: K trips per captured HIP graph), mi.lu (fp64 |
      fp32_ir: FP32 factors with FP64-residual refinement), mi.ir_sweeps (3), mi.lu_form
      (coop: 32 lanes per cell, a whole wave above 32 unknowns, default | lane: one lane per

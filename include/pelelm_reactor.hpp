@@ -88,7 +88,8 @@ inline mi_int4 view_int(const A& a) {
 class ReactorBase {
  public:
   // ReactorBase::create -- "ReactorCvode" / "ReactorBDF" (dense analytic-Jacobian BDF on
-  // the GPU) or "ReactorNull" (rY += dt rYsrc, rE += dt rEsrc).
+  // the GPU), "ReactorRK64" (explicit Runge-Kutta sub-steps) or "ReactorNull"
+  // (rY += dt rYsrc, rE += dt rEsrc).
   static std::unique_ptr<ReactorBase> create(const std::string& name) {
     void* h = nullptr;
     if (mi_chem_create(name.c_str(), &h) != MI_OK)

@@ -98,6 +98,19 @@ int orc_react(const orc_mech* m, int ncell, double* rY, const double* rYsrc, dou
 int orc_radau_react(const orc_mech* m, int ncell, double* rY, const double* rYsrc, double* T, double* rE,
                     const double* rEsrc, double dt, double rtol, double atol, int nthreads, long* nsteps);
 
+/* The explicit reactor (ReactorRK64, rk64.cpp): six-stage Cash-Karp pair, fourth-order
+   solution propagated, sub-steps in [dt/nsub_max, dt/nsub_min] starting at dt/nsub_guess;
+   a cell that fails the error test at the smallest sub-step fails (FC = -evaluations).
+   Returns the number of failed cells. */
+int orc_rk64_react(const orc_mech* m, int ncell, double* rY, const double* rYsrc, double* T, double* rE,
+                   const double* rEsrc, double* FC, const int* mask, double dt, double rtol, double atol,
+                   int nsub_guess, int nsub_min, int nsub_max, int nthreads);
+/* The same integrator on y' = A y; st[0..2] = steps, evaluations, rejections. */
+int orc_rk64_linear(int n, const double* A, double* y, double tout, double rtol, double atol, int nsub_guess,
+                    int nsub_min, int nsub_max, long* st);
+/* c[6], A[6][5], b4[6], b5[6] of the pair. */
+void orc_rk64_tableau(double* out30);
+
 /* Radau IIA-5 coefficients as radau5.cpp derives them: c[3], A[3][3], e[3], g0. */
 void orc_radau_coefficients(double* out16);
 

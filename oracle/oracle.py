@@ -77,6 +77,13 @@ def lib():
                                           C.c_void_p]
             L.orc_radau_react.restype = C.c_int
             L.orc_radau_coefficients.argtypes = [_dp]
+        L.orc_rk64_react.argtypes = [C.c_void_p, C.c_int, _dp, C.c_void_p, _dp, _dp, C.c_void_p, _dp, C.c_void_p,
+                                     C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_rk64_react.restype = C.c_int
+        L.orc_rk64_linear.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_int, C.c_void_p]
+        L.orc_rk64_linear.restype = C.c_int
+        L.orc_rk64_tableau.argtypes = [_dp]
         _lib = L
     return _lib
 
@@ -180,6 +187,23 @@ class Mech:
             out["stats"] = {f: np.array([getattr(s, f) for s in st]) for f, _ in CellStats._fields_}
         return out
 
+    def rk64_react(self, rY, T, rE, dt, rYsrc=None, rEsrc=None, mask=None, rtol=1e-6, atol=1e-10,
+                   nsub=(10, 5, 500), nthreads=0):
+        """The explicit reactor (ReactorRK64) on SoA cells; nsub = (guess, min, max)."""
+        rY = np.array(rY, dtype=np.float64, order="C", copy=True)
+        T = np.array(T, dtype=np.float64, copy=True)
+        rE = np.array(rE, dtype=np.float64, copy=True)
+        ncell = T.shape[0]
+        FC = np.zeros(ncell)
+        src = None if rYsrc is None else np.ascontiguousarray(rYsrc, dtype=np.float64)
+        esrc = None if rEsrc is None else np.ascontiguousarray(rEsrc, dtype=np.float64)
+        msk = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+        nfail = lib().orc_rk64_react(self._h, ncell, rY, None if src is None else src.ctypes.data, T, rE,
+                                     None if esrc is None else esrc.ctypes.data, FC,
+                                     None if msk is None else msk.ctypes.data, float(dt), float(rtol), float(atol),
+                                     int(nsub[0]), int(nsub[1]), int(nsub[2]), int(nthreads))
+        return dict(rY=rY, T=T, rE=rE, FC=FC, nfail=nfail)
+
     def radau_react(self, rY, T, rE, dt, rtol=1e-12, atol=1e-20, rYsrc=None, rEsrc=None, nthreads=0):
         rY = np.array(rY, dtype=np.float64, order="C", copy=True)
         T = np.array(T, dtype=np.float64, copy=True)
@@ -191,6 +215,22 @@ class Mech:
                                       None if esrc is None else esrc.ctypes.data, float(dt), float(rtol),
                                       float(atol), int(nthreads), None)
         return dict(rY=rY, T=T, rE=rE, nfail=nfail)
+
+
+def rk64_tableau():
+    """(c, A, b4, b5) of the explicit reactor's Cash-Karp pair."""
+    out = np.zeros(6 + 30 + 12)
+    lib().orc_rk64_tableau(out)
+    return out[:6], out[6:36].reshape(6, 5), out[36:42], out[42:48]
+
+
+def rk64_linear(A, y0, tout, rtol=1e-8, atol=1e-12, nsub=(10, 5, 100000)):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    y = np.array(y0, dtype=np.float64, copy=True)
+    st = np.zeros(3, dtype=np.int64)
+    r = lib().orc_rk64_linear(A.shape[0], A.ravel(), y, float(tout), float(rtol), float(atol), nsub[0], nsub[1],
+                              nsub[2], st.ctypes.data)
+    return y, r, st
 
 
 def radau_coefficients():

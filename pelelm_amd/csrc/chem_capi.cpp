@@ -25,6 +25,9 @@ namespace {
 struct Reactor {
   std::string integrator;
   bool null_reactor = false;
+  // ReactorRK64: explicit sub-steps within [dt / rk_max, dt / rk_min] from dt / rk_guess
+  bool rk64 = false;
+  int rk_guess = 10, rk_min = 5, rk_max = 500;
   bool inited = false;
   int reactor_type = 2;
   int mech = MECH_DRM19;
@@ -359,6 +362,31 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
     *time += *dt;
     return MI_OK;
   }
+  if (r->rk64) {   // explicit sub-steps, one lane per cell, one kernel
+    if (fixed)
+      return fail(r, MI_EINVAL, "the no-allocation entry point runs the BDF reactor (ReactorRK64 keeps a workspace "
+                                "slot per cell)");
+    int rc = ensure_device_state(r, a.ncell);
+    if (rc) return rc;
+    a.ws = r->ws;
+    a.iws = r->iws;
+    a.counts = r->counts;
+    a.counters = r->counters;
+    r->stat_counts = r->counts;
+    r->stat_counters = r->counters;
+    r->last_schedule = SCHED_PLAIN;
+    r->last_graph = 0;
+    r->last_trips = 0;
+    r->prof = KernelProf{};
+    HIPCHK(r, launch_counters_reset(r->counters, stream));
+    HIPCHK(r, hipEventRecord(r->ev0, stream));
+    HIPCHK(r, launch_rk64(r->mech, a, r->rk_guess, r->rk_min, r->rk_max, stream));
+    HIPCHK(r, hipEventRecord(r->ev1, stream));
+    r->last_stream = stream;
+    r->have_stats = true;
+    *time += *dt;
+    return failure_code(r, stream);
+  }
   if (fixed) {   // the no-allocation entry point: the persistent schedule in fixed memory
     if (r->null_reactor) {
       HIPCHK(r, launch_null(N, a, stream));
@@ -548,6 +576,8 @@ int mi_chem_create(const char* integrator, void** reactor) {
   r->integrator = name;
   if (name == "ReactorNull") {
     r->null_reactor = true;
+  } else if (name == "ReactorRK64") {
+    r->rk64 = true;
   } else if (name == "ReactorCvode" || name == "ReactorBDF") {
     r->null_reactor = false;
   } else {
@@ -626,6 +656,12 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
     else if (v == "auto") r->schedule = SCHED_AUTO;
     else if (v == "persistent") r->schedule = SCHED_PERSIST;
     else return fail(r, MI_EINVAL, "mi.schedule must be plain, queue, auto or persistent");
+  } else if (k == "ode.rk64_nsubsteps_guess" || k == "ode.rk64_nsubsteps_min" || k == "ode.rk64_nsubsteps_max") {
+    if (!parse_long(value, &l) || l < 1 || l > (1L << 24)) return fail(r, MI_EINVAL, k + " must be 1..2^24");
+    if (k == "ode.rk64_nsubsteps_guess") r->rk_guess = (int)l;
+    else if (k == "ode.rk64_nsubsteps_min") r->rk_min = (int)l;
+    else r->rk_max = (int)l;
+    if (r->rk_min > r->rk_max) return fail(r, MI_EINVAL, "ode.rk64_nsubsteps_min exceeds ode.rk64_nsubsteps_max");
   } else if (k == "mi.order") {
     if (v == "none") r->order_mode = ORDER_NONE;
     else if (v == "rate") r->order_mode = ORDER_RATE;

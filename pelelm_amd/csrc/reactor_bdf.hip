@@ -17,7 +17,10 @@
 // wave-tiled workspace -- element (slot, lane) of wave w at ws[(w*nslot+slot)*64+lane]
 // -- so every access of one slot by a wave is one contiguous 512-byte line.
 // Trips of bounded kernels (evaluation, set-up, LU, solve, step logic) issued by the
-// host, with a fixed cell-to-lane mapping (plain) or compacted work lists (queue).
+// host, with a fixed cell-to-lane mapping (plain; the cells optionally sorted by an
+// estimate of their cost so that a wave's cells finish together) or compacted work lists
+// (queue); or one persistent kernel whose waves keep their cells through every trip and
+// claim new cells from a device counter.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -2360,6 +2363,143 @@ __global__ void null_kernel(const ReactArgs a, int nspec) {
   a.FC.p[cell_off(a.FC, i, j, k)] = 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// ReactorRK64: explicit Runge-Kutta sub-steps for cells whose chemistry is not stiff
+// (PelePhysics' six-stage, fourth-order explicit reactor, selected by chem_integrator =
+// "ReactorRK64", Tools/CMake/BuildPeleLMExe.cmake:124-125).  One lane per cell: the
+// six-stage Cash-Karp pair (Cash & Karp 1990), fourth-order solution propagated, the
+// difference from the fifth-order one as the local error in the WRMS norm of
+// rtol max(|y|, |y_new|) + atol; accepted steps grow by min(5, 0.9 err^-1/5), rejected
+// ones (a non-finite error included) shrink by max(0.2, 0.9 err^-1/4), sub-steps within
+// [dt / nsub_max, dt / nsub_min] from dt / nsub_guess; failing the error test at the
+// smallest sub-step fails the cell (FC = -evaluations).  The stage derivatives live in
+// the cell's Nordsieck slots of the workspace (6 n words), the forcing in its FY slot.
+// ---------------------------------------------------------------------------
+struct RK64Opts {
+  int nsub_guess, nsub_min, nsub_max;
+};
+
+__device__ constexpr double RK_C[6] = {0.0, 1.0 / 5.0, 3.0 / 10.0, 3.0 / 5.0, 1.0, 7.0 / 8.0};
+__device__ constexpr double RK_A[6][5] = {
+    {0, 0, 0, 0, 0},
+    {1.0 / 5.0, 0, 0, 0, 0},
+    {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0},
+    {3.0 / 10.0, -9.0 / 10.0, 6.0 / 5.0, 0, 0},
+    {-11.0 / 54.0, 5.0 / 2.0, -70.0 / 27.0, 35.0 / 27.0, 0},
+    {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
+__device__ constexpr double RK_B4[6] = {2825.0 / 27648.0, 0.0, 18575.0 / 48384.0, 13525.0 / 55296.0,
+                                        277.0 / 14336.0, 1.0 / 4.0};
+__device__ constexpr double RK_B5[6] = {37.0 / 378.0, 0.0, 250.0 / 621.0, 125.0 / 594.0, 0.0, 512.0 / 1771.0};
+
+template <class M>
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_EVAL_WAVES(M::N1)) k_rk64(const ReactArgs a,
+                                                                                       const RK64Opts o) {
+  constexpr int N = M::NS, N1 = M::N1;
+  using S = Slots<N1>;
+  const long c = (long)blockIdx.x * 64 + threadIdx.x;
+  unsigned long long t_cells = 0, t_nst = 0, t_nfe = 0, t_rej = 0, t_fail = 0, t_hy2t = 0;
+  if (c < a.ncell) {
+    int ci, cj, ck;
+    cell_ijk(a, c, ci, cj, ck);
+    const bool masked = a.mask.p && a.mask.p[cell_off(a.mask, ci, cj, ck)] < 0;
+    if (masked) {
+      a.FC.p[cell_off(a.FC, ci, cj, ck)] = 0.0;
+    } else {
+      const WS w = cell_ws<M>(a.ws, c);
+      double y[N1];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        y[k] = a.rY.p[cell_off(a.rY, ci, cj, ck) + k * a.rY.ns];
+        w[S::FY + k] = a.rYsrc.p ? a.rYsrc.p[cell_off(a.rYsrc, ci, cj, ck) + k * a.rYsrc.ns] : 0.0;
+      }
+      y[N] = a.T.p[cell_off(a.T, ci, cj, ck)];
+      const double rE0 = a.rE.p[cell_off(a.rE, ci, cj, ck)];
+      const double fH = a.rEsrc.p ? a.rEsrc.p[cell_off(a.rEsrc, ci, cj, ck)] : 0.0;
+      const double dt = a.dt, hmin = dt / o.nsub_max, hmax = dt / o.nsub_min;
+      double h = dt / o.nsub_guess, t = 0.0;
+      long nst = 0, nfe = 0, nrej = 0;
+      const long max_attempts = 4L * o.nsub_max + 100;
+      bool ok = true;
+#pragma unroll 1
+      while (t < dt) {
+        if (nst + nrej >= max_attempts) { ok = false; break; }
+        bool last = false;
+        if (t + h >= dt) { h = dt - t; last = true; }
+        // one call site of the right-hand side for the six stages
+#pragma unroll 1
+        for (int s = 0; s < 6; ++s) {
+          double yt[N1], f[N1], Tsol;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) yt[i] = 0.0;
+#pragma unroll 1
+          for (int j = 0; j < s; ++j) {
+            const double asj = RK_A[s][j];
+#pragma unroll
+            for (int i = 0; i < N1; ++i) yt[i] += asj * w[S::ZN + j * N1 + i];
+          }
+#pragma unroll
+          for (int i = 0; i < N1; ++i) yt[i] = y[i] + h * yt[i];
+          int it = 0;
+          rhs<M>(yt, t + RK_C[s] * h, w, rE0, fH, f, Tsol, &it);
+          t_hy2t += (unsigned long long)it;
+          ++nfe;
+#pragma unroll
+          for (int i = 0; i < N1; ++i) w[S::ZN + s * N1 + i] = f[i];
+        }
+        double en = 0.0, y4[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          double a4 = 0.0, e = 0.0;
+#pragma unroll
+          for (int s = 0; s < 6; ++s) {
+            const double k = w[S::ZN + s * N1 + i];
+            a4 += RK_B4[s] * k;
+            e += (RK_B5[s] - RK_B4[s]) * k;
+          }
+          y4[i] = y[i] + h * a4;
+          const double sc = a.rtol * fmax(fabs(y[i]), fabs(y4[i])) + a.atol[i];
+          const double r = h * e / sc;
+          en += r * r;
+        }
+        en = sqrt(en / N1);
+        if (!isfinite(en)) en = HUGE_VAL;
+        if (en <= 1.0) {
+#pragma unroll
+          for (int i = 0; i < N1; ++i) y[i] = y4[i];
+          t = last ? dt : t + h;
+          ++nst;
+          h = fmin(hmax, h * (en > 0.0 ? fmin(5.0, 0.9 * pow(en, -0.2)) : 5.0));
+        } else {
+          ++nrej;
+          if (h <= hmin * (1.0 + 1e-12)) { ok = false; break; }
+          h = fmax(hmin, h * fmax(0.2, 0.9 * pow(en, -0.25)));
+        }
+      }
+      if (ok) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) a.rY.p[cell_off(a.rY, ci, cj, ck) + k * a.rY.ns] = y[k];
+        a.T.p[cell_off(a.T, ci, cj, ck)] = y[N];
+        a.rE.p[cell_off(a.rE, ci, cj, ck)] = rE0 + fH * dt;
+      }
+      a.FC.p[cell_off(a.FC, ci, cj, ck)] = ok ? (double)nfe : -(double)nfe;
+      t_cells = 1;
+      t_nst = nst;
+      t_nfe = nfe;
+      t_rej = nrej;
+      t_fail = ok ? 0 : 1;
+    }
+  }
+  if (a.counters) {
+    wave_add(&a.counters[CNT_CELLS], t_cells);
+    wave_add(&a.counters[CNT_NST], t_nst);
+    wave_add(&a.counters[CNT_NFE], t_nfe);
+    wave_add(&a.counters[CNT_NETF], t_rej);
+    wave_add(&a.counters[CNT_NFAIL], t_fail);
+    wave_max(&a.counters[CNT_MAXSTEPS], t_nst);
+    wave_add(&a.counters[CNT_HY2T], t_hy2t);
+  }
+}
+
 #define PELE_TRY(x)                     \
   do {                                  \
     const hipError_t e_ = (x);          \
@@ -2747,6 +2887,12 @@ hipError_t launch_cost_keys_t(const ReactArgs& a, int mode, unsigned int* keys, 
   return hipGetLastError();
 }
 
+template <class M>
+hipError_t launch_rk64_t(const ReactArgs& a, const RK64Opts& o, hipStream_t stream) {
+  hipLaunchKernelGGL(k_rk64<M>, dim3((unsigned)((a.ncell + 63) / 64)), dim3(64), 0, stream, a, o);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 int mech_nspec(int mech) {
@@ -2834,6 +2980,17 @@ hipError_t launch_rates(int mech, const RatesArgs& ra, hipStream_t stream) {
     case MECH_LIDRYER: return launch_rates_t<mech_LiDryer>(ra, stream);
     case MECH_DRM19: return launch_rates_t<mech_drm19>(ra, stream);
     case MECH_DODECANE53: return launch_rates_t<mech_dodecane53>(ra, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_rk64(int mech, const ReactArgs& a, int nsub_guess, int nsub_min, int nsub_max,
+                       hipStream_t stream) {
+  const RK64Opts o{nsub_guess, nsub_min, nsub_max};
+  switch (mech) {
+    case MECH_LIDRYER: return launch_rk64_t<mech_LiDryer>(a, o, stream);
+    case MECH_DRM19: return launch_rk64_t<mech_drm19>(a, o, stream);
+    case MECH_DODECANE53: return launch_rk64_t<mech_dodecane53>(a, o, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -158,6 +158,10 @@ struct RatesArgs {
 };
 hipError_t launch_rates(int mech, const RatesArgs& ra, hipStream_t stream);
 hipError_t launch_null(int nspec, const ReactArgs& a, hipStream_t stream);
+// ReactorRK64 over every cell of a call (explicit sub-steps; the workspace's FY and
+// Nordsieck slots hold the forcing and the stage derivatives)
+hipError_t launch_rk64(int mech, const ReactArgs& a, int nsub_guess, int nsub_min, int nsub_max,
+                       hipStream_t stream);
 // copy a call's rY [nspec][ncell], T and rE [ncell] into flat device arrays
 hipError_t launch_rhoh_to_temp(int mech, const ArrView& rho, const ArrView& rhoY, const ArrView& rhoH,
                                const ArrView& T, int nx, int ny, long ncell, hipStream_t stream);

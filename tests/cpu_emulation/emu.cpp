@@ -1,6 +1,7 @@
 // Runs the reactor kernels' device code serially on the CPU (see hip/hip_runtime.h
 // in this directory) so tests can check the kernel logic against the oracle without a GPU.
 #define PELE_WAVES_PER_EU(n)
+#include <limits>
 #include <vector>
 
 #include "../../pelelm_amd/csrc/reactor_bdf.hip"
@@ -130,4 +131,28 @@ extern "C" int emu_cost_keys(const char* mech, long ncell, double* rY, double* T
   a.dt = dt; a.rtol = rtol;
   for (int i = 0; i <= mech_nspec(m); ++i) a.atol[i] = atol;
   return (int)launch_cost_keys(m, a, mode, keys, vals, nullptr);
+}
+
+// ReactorRK64's kernel over flat cells (with the forcing and mask of emu_set_forcing)
+extern "C" int emu_rk64(const char* mech, long ncell, double* rY, double* T, double* rE, double* FC, double dt,
+                        double rtol, double atol, int guess, int nmin, int nmax, unsigned long long* counters) {
+  const int m = mech_lookup(mech);
+  if (m < 0) return -1;
+  const int N = mech_nspec(m);
+  ReactArgs a{};
+  a.rY.p = rY; a.rY.ns = ncell;
+  a.T.p = T; a.T.ns = ncell;
+  a.rE.p = rE; a.rE.ns = ncell;
+  a.FC.p = FC; a.FC.ns = ncell;
+  if (g_fy) { a.rYsrc.p = const_cast<double*>(g_fy); a.rYsrc.ns = ncell; }
+  if (g_fh) { a.rEsrc.p = const_cast<double*>(g_fh); a.rEsrc.ns = ncell; }
+  a.mask.p = g_mask;
+  a.nx = (int)ncell; a.ny = 1; a.ncell = ncell;
+  a.dt = dt; a.rtol = rtol;
+  for (int i = 0; i <= N; ++i) a.atol[i] = atol;
+  const long nw = (ncell + 63) / 64;
+  std::vector<double> ws((size_t)nw * ws_slots(m) * 64, std::numeric_limits<double>::quiet_NaN());
+  a.ws = ws.data();
+  a.counters = counters;
+  return (int)launch_rk64(m, a, guess, nmin, nmax, nullptr);
 }

@@ -37,10 +37,10 @@ def test_library_exports_every_network_symbol():
 
 def test_create_rejects_unknown_integrator():
     with pytest.raises(reactor.ReactorError):
-        reactor.ReactorBase.create("ReactorRK64")
+        reactor.ReactorBase.create("ReactorArkode")
 
 
-@pytest.mark.parametrize("name", ["ReactorCvode", "ReactorBDF", "ReactorNull"])
+@pytest.mark.parametrize("name", ["ReactorCvode", "ReactorBDF", "ReactorRK64", "ReactorNull"])
 def test_create_init_close(name):
     r = reactor.ReactorBase.create(name)
     r.init(2, 100)

@@ -284,3 +284,77 @@ def test_react_under_forcing_keeps_T_on_its_enthalpy(wl, name):
     for c in range(d["T"].shape[0]):
         Tc, _ = m.hy2t(out["rE"][c] / rho[c], out["rY"][:, c] / rho[c], out["T"][c])
         assert abs(out["T"][c] - Tc) <= 1e-6, (c, out["T"][c], Tc)
+
+
+def test_rk64_tableau_orders():
+    """The explicit reactor's Cash-Karp pair: every rooted-tree condition up to order 4
+    for the propagated weights b4 and up to order 5 for b5, and row sums equal to c."""
+    c, A, b4, b5 = O.rk64_tableau()
+    Af = np.zeros((6, 6))
+    Af[:, :5] = A
+    np.testing.assert_allclose(Af.sum(1), c, atol=1e-15)
+    Ac, Ac2, Ac3 = Af @ c, Af @ c ** 2, Af @ c ** 3
+    AAc, AAc2, AcAc = Af @ Ac, Af @ Ac2, Af @ (c * Ac)
+    AAAc = Af @ AAc
+    order4 = [(np.ones(6), 1), (c, 1 / 2), (c ** 2, 1 / 3), (Ac, 1 / 6), (c ** 3, 1 / 4), (c * Ac, 1 / 8),
+              (Ac2, 1 / 12), (AAc, 1 / 24)]
+    order5 = [(c ** 4, 1 / 5), (c ** 2 * Ac, 1 / 10), (c * Ac2, 1 / 15), (c * AAc, 1 / 30), (Ac * Ac, 1 / 20),
+              (Ac3, 1 / 20), (AcAc, 1 / 40), (AAc2, 1 / 60), (AAAc, 1 / 120)]
+    for phi, val in order4:
+        assert abs(b4 @ phi - val) < 1e-14 and abs(b5 @ phi - val) < 1e-14
+    for phi, val in order5:
+        assert abs(b5 @ phi - val) < 1e-14
+    # b4 is only fourth order: at least one fifth-order condition fails
+    assert max(abs(b4 @ phi - val) for phi, val in order5) > 1e-4
+
+
+def test_rk64_integrates_a_linear_system_to_its_tolerance():
+    """y' = A y with a non-stiff A against expm: with the local error held to the tolerance,
+    a fourth-order method's global error falls as rtol^(4/5), about 40x per 100x in rtol
+    (measured 50x), and stays within a few hundred local tolerances over the 20-120 steps."""
+    from scipy.linalg import expm
+    A = np.array([[-1.0, 0.5, 0.0], [0.2, -2.0, 0.3], [0.0, 0.4, -0.5]])
+    y0 = np.array([1.0, 2.0, -1.0])
+    exact = expm(A * 3.0) @ y0
+    errs = []
+    for rtol in (1e-4, 1e-6, 1e-8):
+        y, r, st = O.rk64_linear(A, y0, 3.0, rtol=rtol, atol=rtol * 1e-3, nsub=(10, 1, 100000))
+        assert r == 0 and st[1] == 6 * (st[0] + st[2])
+        errs.append(np.max(np.abs(y - exact) / (np.abs(exact) + 1e-3)))
+    assert errs[0] / errs[1] > 20 and errs[1] / errs[2] > 20
+    assert errs[2] < 300 * 1e-8
+
+
+@pytest.mark.parametrize("wl", ["lidryer_16", "drm19_64"])
+def test_rk64_matches_the_tight_bdf_on_cells_it_can_integrate(wl):
+    """The explicit reactor on non-stiff-enough cells (dt = 1e-6 s) at rtol 1e-8 agrees with
+    the tight BDF to a few tolerance units, conserves mass and rho h with forcing."""
+    n = 64
+    d = cells.generate_workload(wl, ncell=n)
+    m = O.Mech(d["mech"])
+    rng = np.random.default_rng(4)
+    fY = rng.normal(size=d["rY"].shape) * 1e-3 * d["rY"] / d["dt"]
+    fH = rng.normal(size=n) * 1e-3 * np.abs(d["rE"]) / d["dt"]
+    r = m.rk64_react(d["rY"], d["T"], d["rE"], d["dt"], rYsrc=fY, rEsrc=fH, rtol=1e-8, atol=1e-14,
+                     nsub=(10, 5, 100000))
+    tight = m.react(d["rY"], d["T"], d["rE"], d["dt"], rYsrc=fY, rEsrc=fH, rtol=1e-12, atol=1e-20, max_order=5)
+    assert r["nfail"] == 0 and tight["nfail"] == 0
+    tol = 1e-8 * np.abs(tight["rY"]) + 1e-14
+    assert np.max(np.abs(r["rY"] - tight["rY"]) / tol) < 20.0
+    assert np.max(np.abs(r["T"] - tight["T"])) < 1e-4
+    np.testing.assert_allclose(r["rY"].sum(0), d["rY"].sum(0) + d["dt"] * fY.sum(0), rtol=1e-12)
+    np.testing.assert_allclose(r["rE"], d["rE"] + d["dt"] * fH, rtol=1e-15)
+
+
+def test_rk64_fails_stiff_cells_within_its_substep_budget():
+    """Stiff flame-front cells over dt = 1e-5 s need more than 500 explicit sub-steps: those
+    cells fail (FC < 0, state unchanged); with a larger budget they integrate."""
+    n = 128
+    d = cells.generate_workload("drm19_128_mix", ncell=n)
+    m = O.Mech("drm19")
+    r = m.rk64_react(d["rY"], d["T"], d["rE"], d["dt"], nsub=(10, 5, 500))
+    failed = r["FC"] < 0
+    assert 0 < failed.sum() < n and r["nfail"] == failed.sum()
+    np.testing.assert_array_equal(r["rY"][:, failed], d["rY"][:, failed])
+    r2 = m.rk64_react(d["rY"], d["T"], d["rE"], d["dt"], nsub=(10, 5, 20000))
+    assert r2["nfail"] == 0
