@@ -73,9 +73,11 @@ typedef struct {
   double kernel_ms[MI_NKERNEL_KINDS];
   long kernel_launches[MI_NKERNEL_KINDS];
   long kernel_cells[MI_NKERNEL_KINDS];
-  /* how the trips ran: schedule 0 plain / 1 queue, graph_trips 0 host-driven or K trips
-     per captured graph; with mi.schedule = auto, the trial times (ms) of plain, queue,
-     plain + graphs and queue + graphs for this box size (0 when not tuned) */
+  /* how the trips ran: schedule 0 plain / 1 queue / 3 persistent, graph_trips 0
+     host-driven or K trips per captured graph; with mi.schedule = auto, for this box size:
+     the chosen configuration's final-round time (ms), the whole search's time (ms), the
+     candidates searched, and 1 when the choice came from mi.tune_cache (all 0 when not
+     tuned) */
   long schedule_used;
   long graph_trips_used;
   double tune_ms[4];
@@ -103,8 +105,12 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
      cvode.solve_type (denseAJ_direct), peleLM.use_typ_vals_chem (0/1),
      mi.mechanism (drm19 | LiDryer | dodecane53), mi.schedule (plain: fixed cell-to-lane mapping
      for the evaluation and step kernels, default | queue: finished cells leave the work
-     lists | auto: the first call of each box size times both, with and without graph
-     capture, on a copy of its cells and keeps the fastest | persistent: one kernel per
+     lists | auto: the first call of each box size searches the plain and queue schedules
+     (host-driven or in graphs of 8 trips, with cost ordering, plain also without) and the
+     persistent one (claim chunks 1, 8, 32; 1 or 2 waves per SIMD) by successive halving on
+     every 16th, every 4th and then every cell of a copy of the call's cells, and keeps the
+     fastest; mi.tune_cache = <file> keeps the choice in a JSON file keyed by mechanism,
+     GPU, tolerances and box size for later processes | persistent: one kernel per
      call, waves keep their cells through all trips and claim new ones from a device
      counter), mi.persist_waves (0: as many as the device holds resident), mi.persist_chunk
      (8: idle lanes of a wave that trigger a claim, 1..64), mi.persist_occupancy (1 | 2

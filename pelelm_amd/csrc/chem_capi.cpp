@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <cmath>
@@ -21,6 +23,15 @@
 using namespace pelelm;
 
 namespace {
+
+// one configuration of the integration among those the autotuner (mi.schedule = auto)
+// chooses from
+struct TuneConfig {
+  int schedule = SCHED_PLAIN;   // SCHED_PLAIN, SCHED_COMPACT or SCHED_PERSIST
+  int graph_trips = 0;          // trip schedules: 0 (host-driven) or K trips per graph
+  int order = ORDER_RATE;       // trip schedules: cost ordering
+  int chunk = 8, occ = 1;       // persistent schedule: claim chunk, waves per SIMD
+};
 
 struct Reactor {
   std::string integrator;
@@ -60,11 +71,14 @@ struct Reactor {
   void* order_temp = nullptr;
   size_t order_temp_bytes = 0;
   long order_cells = 0;
-  // mi.schedule = auto: the configuration chosen per box size, and the trial timings
+  // mi.schedule = auto: the configuration chosen per box size (how it was found and what
+  // it cost: final-round ms of the choice, total tuning ms, candidates, 1 when read from
+  // the cache file), and the cache file (mi.tune_cache; empty: none)
   struct Tuned {
-    int schedule, graph_trips;
+    TuneConfig cfg;
     double ms[4];
   };
+  std::string tune_cache;
   std::map<long, Tuned> tuned;
   int last_schedule = SCHED_COMPACT, last_graph = 0;
   // a cell that exhausts every fallback: abort (default) makes react return MI_ECELLFAIL,
@@ -193,72 +207,6 @@ bool parse_long(const char* s, long* out) {
   return true;
 }
 
-// mi.schedule = auto: time the plain and compacted schedules, each with host-driven and
-// graph-captured trips, on a flat copy of this call's cells (the call's own arrays are
-// untouched) and remember the fastest for this box size.  One-time cost per box size:
-// four integrations of the box.
-int autotune(Reactor* r, const ReactArgs& a0, hipStream_t stream, long max_trips) {
-  const int N = mech_nspec(r->mech);
-  const long n = a0.ncell;
-  double *rY = nullptr, *T = nullptr, *rE = nullptr, *FC = nullptr, *cY = nullptr, *cT = nullptr, *cE = nullptr;
-  struct Free {
-    double** p[7];
-    ~Free() {
-      for (auto q : p)
-        if (*q) (void)hipFree(*q);
-    }
-  } fr{{&rY, &T, &rE, &FC, &cY, &cT, &cE}};
-  HIPCHK(r, hipMalloc(&rY, sizeof(double) * N * n));
-  HIPCHK(r, hipMalloc(&cY, sizeof(double) * N * n));
-  for (double** p : {&T, &rE, &FC, &cT, &cE}) HIPCHK(r, hipMalloc(p, sizeof(double) * n));
-  HIPCHK(r, launch_gather_state(N, a0, cY, cT, cE, stream));
-  ReactArgs a = a0;
-  auto flat = [n](double* p) {
-    ArrView v{};
-    v.p = p;
-    v.js = n;
-    v.ks = n;
-    v.ns = n;
-    return v;
-  };
-  a.rY = flat(rY);
-  a.T = flat(T);
-  a.rE = flat(rE);
-  a.FC = flat(FC);
-  a.rYsrc = ArrView{};
-  a.rEsrc = ArrView{};
-  a.mask = IntView{};
-  a.nx = (int)n;
-  a.ny = 1;
-  a.use_pred = 0;
-  const int sched[4] = {SCHED_PLAIN, SCHED_COMPACT, SCHED_PLAIN, SCHED_COMPACT};
-  const int gk[4] = {0, 0, 8, 8};
-  Reactor::Tuned t{};
-  int best = 0;
-  for (int v = 0; v < 4; ++v) {
-    HIPCHK(r, hipMemcpyAsync(rY, cY, sizeof(double) * N * n, hipMemcpyDeviceToDevice, stream));
-    HIPCHK(r, hipMemcpyAsync(T, cT, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
-    HIPCHK(r, hipMemcpyAsync(rE, cE, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
-    a.compact = (sched[v] == SCHED_COMPACT) ? 1 : 0;
-    TripPlan plan;
-    plan.graph_trips = gk[v];
-    plan.capture_stream = r->capture_stream;
-    long trips = 0;
-    HIPCHK(r, hipEventRecord(r->ev0, stream));
-    HIPCHK(r, run_react(r->mech, a, stream, max_trips, &trips, nullptr, nullptr, plan));
-    HIPCHK(r, hipEventRecord(r->ev1, stream));
-    HIPCHK(r, hipEventSynchronize(r->ev1));
-    float ms = 0.f;
-    HIPCHK(r, hipEventElapsedTime(&ms, r->ev0, r->ev1));
-    t.ms[v] = ms;
-    if (ms < t.ms[best]) best = v;
-  }
-  t.schedule = sched[best];
-  t.graph_trips = gk[best];
-  r->tuned[n] = t;
-  return MI_OK;
-}
-
 // the return code of a finished call under mi.on_fail: one read of the failure counter
 // (the only synchronisation of a persistent call, and none under continue)
 int failure_code(Reactor* r, hipStream_t stream) {
@@ -286,10 +234,10 @@ struct PersistMem {
 // size); the kernels are enqueued on `stream` with no host synchronisation.  With `fixed`
 // the slots and counters are that memory and nothing is allocated.
 int react_persist(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream,
-                  const PersistMem* fixed = nullptr) {
+                  const PersistMem* fixed = nullptr, const TuneConfig* cfg = nullptr) {
   PersistPlan plan;
-  plan.occupancy = r->persist_occ;
-  plan.chunk = r->persist_chunk;
+  plan.occupancy = cfg ? cfg->occ : r->persist_occ;
+  plan.chunk = cfg ? cfg->chunk : r->persist_chunk;
   int& cap = r->persist_cap[r->mech][plan.occupancy - 1];
   if (cap <= 0) cap = persist_capacity(r->mech, plan.occupancy);
   if (cap <= 0) return fail(r, MI_EHIP, "cannot query the persistent kernel's occupancy");
@@ -335,6 +283,254 @@ int react_persist(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_
   r->last_trips = -1;   // read from the device by mi_chem_get_stats
   *time += *dt;
   return capturing ? MI_OK : failure_code(r, stream);
+}
+
+// ---------------------------------------------------------------------------
+// mi.schedule = auto: the autotuner.  Candidates: the plain and compacted trip schedules,
+// host-driven or in graphs of 8 trips, with cost ordering (and plain without it), and the
+// persistent schedule at three claim chunks and two occupancies.  Successive halving on
+// samples of the call's own cells (every 16th cell, then every 4th, then all of them;
+// 3 and then 2 survivors): about three box integrations, once per box size.  The choice is
+// kept per box size and, with mi.tune_cache = <file>, in a JSON file keyed by mechanism,
+// GPU, tolerances and box size, so later processes skip the search.
+// ---------------------------------------------------------------------------
+std::vector<TuneConfig> tune_candidates() {
+  std::vector<TuneConfig> c;
+  for (int sched : {SCHED_PLAIN, SCHED_COMPACT})
+    for (int g : {0, 8}) {
+      TuneConfig t;
+      t.schedule = sched;
+      t.graph_trips = g;
+      t.order = ORDER_RATE;
+      c.push_back(t);
+    }
+  TuneConfig u;
+  u.order = ORDER_NONE;
+  c.push_back(u);
+  for (int chunk : {1, 8, 32}) {
+    TuneConfig t;
+    t.schedule = SCHED_PERSIST;
+    t.chunk = chunk;
+    t.occ = 1;
+    c.push_back(t);
+  }
+  TuneConfig t2;
+  t2.schedule = SCHED_PERSIST;
+  t2.chunk = 8;
+  t2.occ = 2;
+  c.push_back(t2);
+  return c;
+}
+
+const char* sched_name(int s) { return s == SCHED_COMPACT ? "queue" : s == SCHED_PERSIST ? "persistent" : "plain"; }
+const char* order_name(int o) { return o == ORDER_RATE ? "rate" : o == ORDER_FC ? "fc" : "none"; }
+
+std::string tune_key(Reactor* r, long n) {
+  hipDeviceProp_t prop{};
+  int dev = 0;
+  std::string gpu = "unknown";
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    gpu = std::string(prop.gcnArchName) + " " + std::to_string(prop.multiProcessorCount) + "CU";
+  char buf[128];
+  std::snprintf(buf, sizeof(buf), "|%.6g|%.6g|%d|%ld", r->rtol, r->atol, r->use_typ ? 1 : 0, n);
+  return mech_name(r->mech) + std::string("|") + gpu + buf;
+}
+
+// the cache file's entries: key -> the object's text
+std::map<std::string, std::string> cache_read(const std::string& path) {
+  std::map<std::string, std::string> out;
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return out;
+  std::string txt;
+  char buf[4096];
+  size_t got;
+  while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) txt.append(buf, got);
+  std::fclose(f);
+  size_t pos = 0;
+  for (;;) {
+    const size_t k0 = txt.find('"', pos);
+    if (k0 == std::string::npos) break;
+    const size_t k1 = txt.find('"', k0 + 1);
+    const size_t o0 = txt.find('{', k1);
+    const size_t o1 = txt.find('}', o0);
+    if (k1 == std::string::npos || o0 == std::string::npos || o1 == std::string::npos) break;
+    out[txt.substr(k0 + 1, k1 - k0 - 1)] = txt.substr(o0, o1 - o0 + 1);
+    pos = o1 + 1;
+  }
+  return out;
+}
+
+std::string json_field(const std::string& obj, const char* name) {
+  const std::string key = std::string("\"") + name + "\":";
+  size_t p = obj.find(key);
+  if (p == std::string::npos) return "";
+  p += key.size();
+  while (p < obj.size() && (obj[p] == ' ' || obj[p] == '"')) ++p;
+  size_t e = p;
+  while (e < obj.size() && obj[e] != '"' && obj[e] != ',' && obj[e] != '}') ++e;
+  return obj.substr(p, e - p);
+}
+
+bool cache_lookup(const std::string& path, const std::string& key, TuneConfig* cfg, double* ms) {
+  const auto entries = cache_read(path);
+  const auto it = entries.find(key);
+  if (it == entries.end()) return false;
+  const std::string& o = it->second;
+  const std::string sc = json_field(o, "schedule"), od = json_field(o, "order");
+  TuneConfig c;
+  if (sc == "plain") c.schedule = SCHED_PLAIN;
+  else if (sc == "queue") c.schedule = SCHED_COMPACT;
+  else if (sc == "persistent") c.schedule = SCHED_PERSIST;
+  else return false;
+  c.order = od == "rate" ? ORDER_RATE : od == "fc" ? ORDER_FC : ORDER_NONE;
+  c.graph_trips = std::atoi(json_field(o, "graph_trips").c_str());
+  c.chunk = std::atoi(json_field(o, "persist_chunk").c_str());
+  c.occ = std::atoi(json_field(o, "persist_occupancy").c_str());
+  if (c.graph_trips < 0 || (c.graph_trips & 1) || c.chunk < 1 || c.chunk > 64 || (c.occ != 1 && c.occ != 2))
+    return false;
+  *cfg = c;
+  *ms = std::atof(json_field(o, "ms").c_str());
+  return true;
+}
+
+void cache_store(const std::string& path, const std::string& key, const TuneConfig& c, double ms) {
+  auto entries = cache_read(path);
+  char buf[256];
+  std::snprintf(buf, sizeof(buf),
+                "{\"schedule\": \"%s\", \"graph_trips\": %d, \"order\": \"%s\", \"persist_chunk\": %d, "
+                "\"persist_occupancy\": %d, \"ms\": %.4f}",
+                sched_name(c.schedule), c.graph_trips, order_name(c.order), c.chunk, c.occ, ms);
+  entries[key] = buf;
+  const std::string tmp = path + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "w");
+  if (!f) return;
+  std::fprintf(f, "{\n");
+  size_t i = 0;
+  for (const auto& e : entries)
+    std::fprintf(f, " \"%s\": %s%s\n", e.first.c_str(), e.second.c_str(), ++i < entries.size() ? "," : "");
+  std::fprintf(f, "}\n");
+  std::fclose(f);
+  (void)std::rename(tmp.c_str(), path.c_str());
+}
+
+// one trial integration of the flat cells of `a` (the reactor's workspace) in
+// configuration c, timed with the reactor's events
+int trial(Reactor* r, ReactArgs a, const TuneConfig& c, hipStream_t stream, long max_trips, float* ms) {
+  HIPCHK(r, launch_counters_reset(r->counters, stream));
+  HIPCHK(r, hipEventRecord(r->ev0, stream));
+  if (c.schedule == SCHED_PERSIST) {
+    int& cap = r->persist_cap[r->mech][c.occ - 1];
+    if (cap <= 0) cap = persist_capacity(r->mech, c.occ);
+    if (cap <= 0) return fail(r, MI_EHIP, "cannot query the persistent kernel's occupancy");
+    PersistPlan plan;
+    plan.occupancy = c.occ;
+    plan.chunk = c.chunk;
+    plan.waves = (int)std::max(1L, std::min((long)cap, (a.ncell + 63) / 64));
+    a.compact = 0;
+    HIPCHK(r, run_react_persist(r->mech, a, plan, max_trips, stream));
+  } else {
+    a.compact = c.schedule == SCHED_COMPACT ? 1 : 0;
+    a.slot_cell = nullptr;
+    if (c.order != ORDER_NONE) {
+      const int rc = ensure_order_buffers(r, a.ncell);
+      if (rc) return rc;
+      HIPCHK(r, launch_cost_keys(r->mech, a, c.order, r->order_keys, r->order_vals, stream));
+      const int* sorted = nullptr;
+      HIPCHK(r, cost_sort(r->order_temp, r->order_temp_bytes, r->order_keys, r->order_vals, a.ncell, stream,
+                          &sorted));
+      a.slot_cell = sorted;
+    }
+    TripPlan plan;
+    plan.graph_trips = c.graph_trips;
+    plan.capture_stream = r->capture_stream;
+    long trips = 0;
+    HIPCHK(r, run_react(r->mech, a, stream, max_trips, &trips, nullptr, nullptr, plan));
+  }
+  HIPCHK(r, hipEventRecord(r->ev1, stream));
+  HIPCHK(r, hipEventSynchronize(r->ev1));
+  HIPCHK(r, hipEventElapsedTime(ms, r->ev0, r->ev1));
+  return MI_OK;
+}
+
+// the search; a0 carries the reactor's workspace (sized for the call's cells)
+int autotune(Reactor* r, const ReactArgs& a0, hipStream_t stream, long max_trips) {
+  const int N = mech_nspec(r->mech);
+  const long n = a0.ncell;
+  Reactor::Tuned t{};
+  const std::string key = tune_key(r, n);
+  if (!r->tune_cache.empty() && cache_lookup(r->tune_cache, key, &t.cfg, &t.ms[0])) {
+    t.ms[1] = 0.0;
+    t.ms[2] = 0.0;
+    t.ms[3] = 1.0;
+    r->tuned[n] = t;
+    return MI_OK;
+  }
+  const auto t_start = std::chrono::steady_clock::now();
+  double *rY = nullptr, *T = nullptr, *rE = nullptr, *FC = nullptr, *cY = nullptr, *cT = nullptr, *cE = nullptr,
+         *fY = nullptr, *fE = nullptr;
+  struct Free {
+    double** p[9];
+    ~Free() {
+      for (auto q : p)
+        if (*q) (void)hipFree(*q);
+    }
+  } fr{{&rY, &T, &rE, &FC, &cY, &cT, &cE, &fY, &fE}};
+  for (double** p : {&rY, &cY, &fY}) HIPCHK(r, hipMalloc(p, sizeof(double) * N * n));
+  for (double** p : {&T, &rE, &FC, &cT, &cE, &fE}) HIPCHK(r, hipMalloc(p, sizeof(double) * n));
+  std::vector<TuneConfig> cand = tune_candidates();
+  const long strides[3] = {16, 4, 1};
+  const size_t keep[3] = {3, 2, 1};
+  std::vector<float> last_ms;
+  for (int round = 0; round < 3; ++round) {
+    const long m = (n + strides[round] - 1) / strides[round];
+    HIPCHK(r, launch_gather_state(N, a0, strides[round], m, cY, cT, cE, fY, fE, stream));
+    ReactArgs a = a0;
+    auto flat = [m](double* p) {
+      ArrView v{};
+      v.p = p;
+      v.js = m;
+      v.ks = m;
+      v.ns = m;
+      return v;
+    };
+    a.rY = flat(rY);
+    a.T = flat(T);
+    a.rE = flat(rE);
+    a.FC = flat(FC);
+    a.rYsrc = flat(fY);
+    a.rEsrc = flat(fE);
+    a.mask = IntView{};
+    a.nx = (int)m;
+    a.ny = 1;
+    a.ncell = m;
+    a.use_pred = 0;
+    std::vector<std::pair<float, size_t>> res;
+    for (size_t i = 0; i < cand.size(); ++i) {
+      HIPCHK(r, hipMemcpyAsync(rY, cY, sizeof(double) * N * m, hipMemcpyDeviceToDevice, stream));
+      HIPCHK(r, hipMemcpyAsync(T, cT, sizeof(double) * m, hipMemcpyDeviceToDevice, stream));
+      HIPCHK(r, hipMemcpyAsync(rE, cE, sizeof(double) * m, hipMemcpyDeviceToDevice, stream));
+      float ms = 0.f;
+      const int rc = trial(r, a, cand[i], stream, max_trips, &ms);
+      if (rc) return rc;
+      res.push_back({ms, i});
+    }
+    std::stable_sort(res.begin(), res.end());
+    std::vector<TuneConfig> next;
+    last_ms.clear();
+    for (size_t i = 0; i < std::min(keep[round], res.size()); ++i) {
+      next.push_back(cand[res[i].second]);
+      last_ms.push_back(res[i].first);
+    }
+    if (round == 0) t.ms[2] = (double)cand.size();
+    cand = next;
+  }
+  t.cfg = cand[0];
+  t.ms[0] = last_ms[0];
+  t.ms[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  t.ms[3] = 0.0;
+  r->tuned[n] = t;
+  if (!r->tune_cache.empty()) cache_store(r->tune_cache, key, t.cfg, t.ms[0]);
+  return MI_OK;
 }
 
 int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t stream,
@@ -418,6 +614,7 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   if (graph_trips > 0 || r->schedule == SCHED_AUTO) {
     if (!r->capture_stream) HIPCHK(r, hipStreamCreateWithFlags(&r->capture_stream, hipStreamNonBlocking));
   }
+  int order_mode = r->order_mode;
   if (r->schedule == SCHED_AUTO && !r->null_reactor) {
     auto it = r->tuned.find(a.ncell);
     if (it == r->tuned.end()) {
@@ -425,8 +622,11 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
       if (rc) return rc;
       it = r->tuned.find(a.ncell);
     }
-    a.compact = (it->second.schedule == SCHED_COMPACT) ? 1 : 0;
-    graph_trips = it->second.graph_trips;
+    const TuneConfig cfg = it->second.cfg;
+    if (cfg.schedule == SCHED_PERSIST) return react_persist(r, a, dt, time, stream, nullptr, &cfg);
+    a.compact = (cfg.schedule == SCHED_COMPACT) ? 1 : 0;
+    graph_trips = cfg.graph_trips;
+    order_mode = cfg.order;
   }
   if (a.use_pred && !a.compact) return fail(r, MI_EINVAL, "the learned predictor needs mi.schedule = queue");
   r->last_schedule = a.compact ? SCHED_COMPACT : SCHED_PLAIN;
@@ -436,12 +636,12 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   if (r->null_reactor) {
     HIPCHK(r, launch_null(N, a, stream));
   } else {
-    if (r->order_mode != ORDER_NONE) {
+    if (order_mode != ORDER_NONE) {
       // workspace slot s integrates cell slot_cell[s], the cells in ascending cost, so a
       // wave's lanes finish together (the keys and the sort are part of the call)
       rc = ensure_order_buffers(r, a.ncell);
       if (rc) return rc;
-      HIPCHK(r, launch_cost_keys(r->mech, a, r->order_mode, r->order_keys, r->order_vals, stream));
+      HIPCHK(r, launch_cost_keys(r->mech, a, order_mode, r->order_keys, r->order_vals, stream));
       const int* sorted = nullptr;
       HIPCHK(r, cost_sort(r->order_temp, r->order_temp_bytes, r->order_keys, r->order_vals, a.ncell, stream,
                           &sorted));
@@ -662,6 +862,8 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
     else if (k == "ode.rk64_nsubsteps_min") r->rk_min = (int)l;
     else r->rk_max = (int)l;
     if (r->rk_min > r->rk_max) return fail(r, MI_EINVAL, "ode.rk64_nsubsteps_min exceeds ode.rk64_nsubsteps_max");
+  } else if (k == "mi.tune_cache") {
+    r->tune_cache = v;
   } else if (k == "mi.order") {
     if (v == "none") r->order_mode = ORDER_NONE;
     else if (v == "rate") r->order_mode = ORDER_RATE;

@@ -3072,21 +3072,29 @@ hipError_t launch_counters_reset(unsigned long long* counters, hipStream_t strea
   return hipGetLastError();
 }
 
-// flat structure-of-arrays copy of a call's rY, T and rE (cell fastest), for trial runs
-__global__ void gather_state_kernel(const ReactArgs a, int nspec, double* __restrict__ rY, double* __restrict__ T,
-                                    double* __restrict__ rE) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.ncell) return;
+// flat structure-of-arrays copy of every stride-th cell of a call (m of them): rY, T, rE
+// and the forcing (zero where the call has none), for the autotuner's trial runs
+__global__ void gather_state_kernel(const ReactArgs a, int nspec, long stride, long m, double* __restrict__ rY,
+                                    double* __restrict__ T, double* __restrict__ rE, double* __restrict__ fY,
+                                    double* __restrict__ fE) {
+  const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= m) return;
+  const long c = o * stride;
   int i, j, k;
   cell_ijk(a, c, i, j, k);
-  for (int s = 0; s < nspec; ++s) rY[s * a.ncell + c] = a.rY.p[cell_off(a.rY, i, j, k) + s * a.rY.ns];
-  T[c] = a.T.p[cell_off(a.T, i, j, k)];
-  rE[c] = a.rE.p[cell_off(a.rE, i, j, k)];
+  for (int s = 0; s < nspec; ++s) {
+    rY[s * m + o] = a.rY.p[cell_off(a.rY, i, j, k) + s * a.rY.ns];
+    fY[s * m + o] = a.rYsrc.p ? a.rYsrc.p[cell_off(a.rYsrc, i, j, k) + s * a.rYsrc.ns] : 0.0;
+  }
+  T[o] = a.T.p[cell_off(a.T, i, j, k)];
+  rE[o] = a.rE.p[cell_off(a.rE, i, j, k)];
+  fE[o] = a.rEsrc.p ? a.rEsrc.p[cell_off(a.rEsrc, i, j, k)] : 0.0;
 }
 
-hipError_t launch_gather_state(int nspec, const ReactArgs& a, double* rY, double* T, double* rE, hipStream_t stream) {
-  hipLaunchKernelGGL(gather_state_kernel, dim3((unsigned)((a.ncell + 255) / 256)), dim3(256), 0, stream, a, nspec,
-                     rY, T, rE);
+hipError_t launch_gather_state(int nspec, const ReactArgs& a, long stride, long m, double* rY, double* T,
+                               double* rE, double* fY, double* fE, hipStream_t stream) {
+  hipLaunchKernelGGL(gather_state_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, a, nspec, stride,
+                     m, rY, T, rE, fY, fE);
   return hipGetLastError();
 }
 

@@ -169,6 +169,9 @@ hipError_t launch_glue_in(const ArrView& sold, const ArrView& snew, const ArrVie
                           int nx, int ny, long ncell, hipStream_t stream);
 hipError_t launch_glue_out(const ArrView& sold, const ArrView& snew, const ArrView& force, const ArrView& rdot,
                            double dt_inv, int nspec, int nx, int ny, long ncell, hipStream_t stream);
-hipError_t launch_gather_state(int nspec, const ReactArgs& a, double* rY, double* T, double* rE, hipStream_t stream);
+// every stride-th cell of a call (m of them) as flat arrays, forcing included (zeros
+// where the call has none)
+hipError_t launch_gather_state(int nspec, const ReactArgs& a, long stride, long m, double* rY, double* T,
+                               double* rE, double* fY, double* fE, hipStream_t stream);
 
 }  // namespace pelelm

@@ -301,28 +301,42 @@ def test_graph_captured_trips_match_oracle(schedule, wl, mech):
     _parity(f"{wl} {schedule} graphs", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)
 
 
-def test_auto_schedule_tunes_once_per_box_size():
-    n = 4096
-    d = cells.generate_workload("lidryer_16", ncell=n)
-    ref = O.Mech("LiDryer").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
-    r = _mk("LiDryer", ode__rtol=1e-6, ode__atol=1e-10, mi__schedule="auto")
-    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
-    tunes = []
-    for call in range(2):
-        rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
-        FC = torch.zeros(n, dtype=torch.float64, device=DEV)
-        r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
-        torch.cuda.synchronize()
-        st = r.stats()
-        assert st["nfail"] == 0
-        _parity(f"lidryer_16 auto call {call}", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)
-        tunes.append(list(st["tune_ms"]))
-        best = int(np.argmin(st["tune_ms"]))
-        assert st["schedule_used"] == best % 2 and st["graph_trips_used"] == (8 if best >= 2 else 0)
-    r.close()
-    assert all(t > 0 for t in tunes[0]) and tunes[0] == tunes[1]   # tuned on the first call only
-    print(f"auto schedule, 4096 LiDryer cells: plain {tunes[0][0]:.2f} ms, queue {tunes[0][1]:.2f} ms, "
-          f"plain+graphs {tunes[0][2]:.2f} ms, queue+graphs {tunes[0][3]:.2f} ms")
+@pytest.mark.parametrize("wl,mech,n", [("lidryer_16", "LiDryer", 4096), ("drm19_128_mix", "drm19", 16384)])
+def test_auto_schedule_tunes_once_per_box_size_and_caches(wl, mech, n, tmp_path):
+    """mi.schedule = auto: the first call of a box size runs the successive-halving search
+    (9 candidates), later calls reuse the choice, the results match the oracle whatever
+    was chosen; with mi.tune_cache the choice goes to a JSON file, and a new reactor with
+    the same file takes it from there without searching."""
+    d = cells.generate_workload(wl, ncell=n)
+    ref = O.Mech(mech).react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    cache = str(tmp_path / "tune.json")
+    chosen = None
+    for reactor_no in range(2):
+        r = _mk(mech, ode__rtol=1e-6, ode__atol=1e-10, mi__schedule="auto", mi__tune_cache=cache)
+        tunes = []
+        for call in range(2):
+            rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+            FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+            r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+            torch.cuda.synchronize()
+            st = r.stats()
+            assert st["nfail"] == 0
+            _parity(f"{wl} auto reactor {reactor_no} call {call}", rY.cpu().numpy(), T.cpu().numpy(),
+                    FC.cpu().numpy(), ref)
+            tunes.append(list(st["tune_ms"]))
+            used = (st["schedule_used"], st["graph_trips_used"])
+            chosen = used if chosen is None else chosen
+            assert used == chosen
+        r.close()
+        assert tunes[0] == tunes[1]   # searched (or read) on the first call only
+        if reactor_no == 0:
+            assert tunes[0][2] == 9 and tunes[0][3] == 0 and tunes[0][1] > tunes[0][0] > 0
+            print(f"auto, {n} {wl} cells: chose schedule {chosen[0]} graphs {chosen[1]}, "
+                  f"{tunes[0][0]:.2f} ms, search {tunes[0][1]:.0f} ms")
+        else:
+            assert tunes[0][3] == 1 and tunes[0][2] == 0   # from the cache file
+    txt = open(cache).read()
+    assert mech in txt and str(n) in txt and '"schedule"' in txt
 
 
 def test_typical_values_scale_the_absolute_tolerances():
