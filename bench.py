@@ -25,6 +25,8 @@ schedule[+modifier...]:
   +fp32  -- FP32 LU of I - gamma J with up to 3 FP64-residual refinement sweeps
   +ir0   -- with +fp32: no refinement sweeps (the Newton iteration's own FP64 residuals
             correct the FP32 solves)
+  +mfma  -- the blocked LU on the matrix cores (mi.lu_form = mfma: 16x16x4 FP64 or FP32
+            tiles for the trailing updates) in place of the cooperative LU
   +graphs -- trips captured into HIP graphs of 8
   +pred  -- the learned Newton predictor, trained before the timed region on pairs the
             plain integrator generates from cells of a seed of their own (setup time
@@ -221,6 +223,8 @@ def run_form(form, mech, dev_in, ncell, nspec, dt, steps, warmup, profile, net=N
         r.set("mi.order", "fc")
     if "ir0" in mods:
         r.set("mi.ir_sweeps", 0)
+    if "mfma" in mods:
+        r.set("mi.lu_form", "mfma")
     r.init(2, ncell)
     if "pred" in mods:
         predictor.attach(r, net)

@@ -122,7 +122,9 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
      issues every trip, default | K even: K trips per captured HIP graph), mi.lu (fp64 |
      fp32_ir: FP32 factors with FP64-residual refinement), mi.ir_sweeps (3), mi.lu_form
      (coop: 32 lanes per cell, a whole wave above 32 unknowns, default | lane: one lane per
-     cell), mi.ir_tol (1e-4: a refinement sweep whose correction has WRMS norm at most this,
+     cell | mfma: one wave per cell, blocked LU in LDS with the trailing updates on 16x16x4
+     FP64 or FP32 matrix-core tiles; the trip schedules only, the persistent kernel factors
+     cooperatively), mi.ir_tol (1e-4: a refinement sweep whose correction has WRMS norm at most this,
      in error-weight units, ends the refinement; a solve that does not get there in
      mi.ir_sweeps sweeps makes its cell refactor in FP64), mi.max_trips (0: 3 x
      cvode.max_num_steps + 100; cells still integrating after this many trips fail),

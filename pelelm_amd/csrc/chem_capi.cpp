@@ -56,6 +56,7 @@ struct Reactor {
   double ir_tol = 1e-4;         // refinement acceptance (WRMS of the correction, error-weight units)
   long max_trips = 0;           // 0: 3 x cvode.max_num_steps + 100 trips
   int lu_lane = 0;
+  int lu_mfma = 0;
   void* predictor = nullptr;    // attached network (mi_chem_attach_predictor), not owned
   int graph_trips = 0;          // 0: host-driven trips; K (even): graphs of K trips
   // mi.schedule = persistent: waves (0: as many as the device holds resident), refill
@@ -550,6 +551,7 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   a.ir_sweeps = r->ir_sweeps;
   a.ir_tol = r->ir_tol;
   a.lu_lane = r->lu_lane;
+  a.lu_mfma = r->lu_mfma;
   a.compact = (r->schedule == SCHED_COMPACT) ? 1 : 0;
   int graph_trips = r->graph_trips;
   a.cell_status = nullptr;
@@ -886,9 +888,10 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
     else if (v == "fp32_ir") r->lu_mode = LU_FP32_IR;
     else return fail(r, MI_EINVAL, "mi.lu must be fp64 or fp32_ir");
   } else if (k == "mi.lu_form") {
-    if (v == "coop") r->lu_lane = 0;
-    else if (v == "lane") r->lu_lane = 1;
-    else return fail(r, MI_EINVAL, "mi.lu_form must be coop or lane");
+    if (v == "coop") { r->lu_lane = 0; r->lu_mfma = 0; }
+    else if (v == "lane") { r->lu_lane = 1; r->lu_mfma = 0; }
+    else if (v == "mfma") { r->lu_lane = 0; r->lu_mfma = 1; }
+    else return fail(r, MI_EINVAL, "mi.lu_form must be coop, lane or mfma");
   } else if (k == "mi.ir_sweeps") {
     if (!parse_long(value, &l) || l < 0 || l > 10) return fail(r, MI_EINVAL, "mi.ir_sweeps must be 0..10");
     r->ir_sweeps = (int)l;

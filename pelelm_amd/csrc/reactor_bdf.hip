@@ -1570,6 +1570,212 @@ __global__ void __launch_bounds__(64) k_lu_coop(const ReactArgs a, const int* __
   }
 }
 
+// ---------------------------------------------------------------------------
+// LU of M = I - gamma J on the matrix cores (mi.lu_form = mfma): one cell per wave, M
+// staged in LDS (NP x NP, NP the unknowns rounded up to 16, 32 or 64), blocked right-
+// looking elimination with panels of four columns.  Within a panel each column's pivot
+// is the largest |m_rk| over the rows not yet pivoted (ties to the lowest row, a NaN as
+// infinite -- the cooperative factorisation's choice), multipliers m_rk (1 / pivot) and
+// the panel's other columns are updated on the vector ALUs, row by lane; the panel's
+// pivot rows then take the updates of the earlier pivots of the panel in the trailing
+// columns (a forward substitution, column by lane, in the unblocked order); and the
+// trailing matrix of the rows not yet pivoted takes the rank-4 update L21 U12 as 16 x 16
+// tiles on v_mfma_f64_16x16x4_f64 (FP64 factors) or v_mfma_f32_16x16x4_f32 (FP32 factors).
+// Rows are not moved: a row ends as row `step` of the factors, the step at which it was
+// chosen (PERM[step] = row), which is the layout the cooperative solves read.
+//
+// The 16 x 16 x 4 operand layout (one block, 64 lanes; measured by a GPU unit test): lane l
+// holds A[l % 16][l / 16] and B[l / 16][l % 16]; its result register v holds
+// D[l / 16 + 4 v][l % 16] for the FP64 product and D[4 (l / 16) + v][l % 16] for the FP32 one.
+// ---------------------------------------------------------------------------
+typedef double pele_double4 __attribute__((ext_vector_type(4)));
+typedef float pele_float4 __attribute__((ext_vector_type(4)));
+
+template <class Real>
+struct MatCore;
+template <>
+struct MatCore<double> {
+  using V4 = pele_double4;
+  static __device__ __forceinline__ V4 mac(double a, double b, V4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // the result row register v of lanes 16 g .. 16 g + 15 holds
+  static __device__ __forceinline__ int drow(int g, int v) { return g + 4 * v; }
+};
+template <>
+struct MatCore<float> {
+  using V4 = pele_float4;
+  static __device__ __forceinline__ V4 mac(float a, float b, V4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int drow(int g, int v) { return 4 * g + v; }
+};
+
+template <int N1>
+struct MfmaDim {
+  static constexpr int NP = N1 <= 16 ? 16 : N1 <= 32 ? 32 : 64;
+  static constexpr int LD = NP + 1;   // LDS row pitch (an odd word count spreads the banks)
+  static_assert(N1 <= 64, "one wave factors at most 64 unknowns");
+};
+
+// factors the cell of `mat` when `work` (all 64 lanes call it); sm: NP x LD Reals of LDS,
+// sdone / sstep: 64 ints each; returns 0, or 1 + the first column without a nonzero pivot
+template <int N1, class Real>
+__device__ __forceinline__ int lu_mfma_core(const CM& mat, const bool work, const double gamma, Real* sm,
+                                            int* sdone, int* sstep) {
+  constexpr int NP = MfmaDim<N1>::NP, LD = MfmaDim<N1>::LD, NT = NP / 16;
+  using A = LUAcc<N1, Real>;
+  using MS = MatSlots<N1>;
+  using V4 = typename MatCore<Real>::V4;
+  const int l = threadIdx.x;
+  if (!work) return 0;   // uniform over the block (one cell per block)
+  // M = I - gamma J from the Jacobian copy, whole lines per load; padding zero
+#pragma unroll 1
+  for (int e = l; e < NP * NP; e += 64) {
+    const int i = e / NP, j = e - (e / NP) * NP;
+    Real v = (Real)0;
+    if (i < N1 && j < N1) v = (Real)((i == j ? 1.0 : 0.0) - gamma * mat[MS::JS + i * N1 + j]);
+    sm[i * LD + j] = v;
+  }
+  bool done = l >= N1;
+  sdone[l] = done ? 1 : 0;
+  int sing = 0;
+  __syncthreads();
+#pragma unroll 1
+  for (int k0 = 0; k0 < N1; k0 += 4) {
+    const int kb = (N1 - k0 < 4) ? N1 - k0 : 4;
+    int prow0 = 0, prow1 = 0, prow2 = 0, prow3 = 0;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      if (kk < kb) {
+        const int k = k0 + kk;
+        unsigned long long key = 0ull;
+        if (!done) {
+          double av = fabs((double)sm[l * LD + k]);
+          if (av != av) av = HUGE_VAL;
+          key = (unsigned long long)__double_as_longlong(av) + 1ull;
+        }
+        unsigned long long kmax = key;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const unsigned long long o = __shfl_xor(kmax, off, 64);
+          kmax = o > kmax ? o : kmax;
+        }
+        const int who = __ffsll((long long)__ballot(key == kmax)) - 1;
+        const bool nonzero = kmax > 1ull;
+        if (!nonzero && sing == 0) sing = k + 1;
+        if (kk == 0) prow0 = who;
+        if (kk == 1) prow1 = who;
+        if (kk == 2) prow2 = who;
+        if (kk == 3) prow3 = who;
+        if (!done && l != who && nonzero) {
+          const Real li = sm[l * LD + k] * ((Real)1 / sm[who * LD + k]);
+          sm[l * LD + k] = li;
+          if (li != (Real)0) {
+#pragma unroll 1
+            for (int c = k + 1; c < k0 + kb; ++c) sm[l * LD + c] -= li * sm[who * LD + c];
+          }
+        }
+        if (l == who && !done) {
+          done = true;
+          sdone[l] = 1;
+          sstep[k] = l;
+        }
+        __syncthreads();
+      }
+    }
+    if (k0 + kb >= N1) break;
+    // the panel's pivot rows in the trailing columns: the updates of the panel's earlier
+    // pivots, in the order the unblocked elimination applies them (column c in lane c)
+    const int prow[4] = {prow0, prow1, prow2, prow3};
+    {
+      const int c = l;
+      if (c >= k0 + kb && c < N1) {
+#pragma unroll
+        for (int kp = 0; kp < 3; ++kp)
+#pragma unroll
+          for (int kk = kp + 1; kk < 4; ++kk)
+            if (kk < kb) {
+              const Real lf = sm[prow[kk] * LD + k0 + kp];
+              if (lf != (Real)0) sm[prow[kk] * LD + c] -= lf * sm[prow[kp] * LD + c];
+            }
+      }
+    }
+    __syncthreads();
+    // trailing rows not yet pivoted: M_rc -= sum_kk L_r,k0+kk U_kk,c on the matrix cores
+    {
+      const int kk = l >> 4, lo = l & 15;
+      const int tj0 = (k0 + kb) / 16;
+      const int prk = prow[kk];
+#pragma unroll 1
+      for (int ti = 0; ti < NT; ++ti) {
+        const int arow = ti * 16 + lo;
+        const Real av = (kk < kb && arow < N1 && sdone[arow] == 0) ? -sm[arow * LD + k0 + kk] : (Real)0;
+#pragma unroll 1
+        for (int tj = tj0; tj < NT; ++tj) {
+          const int col = tj * 16 + lo;
+          const bool colok = col >= k0 + kb && col < N1;
+          const Real bv = (kk < kb && colok) ? sm[prk * LD + col] : (Real)0;
+          V4 cv;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) cv[v] = sm[(ti * 16 + MatCore<Real>::drow(kk, v)) * LD + col];
+          cv = MatCore<Real>::mac(av, bv, cv);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = ti * 16 + MatCore<Real>::drow(kk, v);
+            if (colok && row < N1 && sdone[row] == 0) sm[row * LD + col] = cv[v];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // rows to their factor rows: factor row s is matrix row sstep[s]
+#pragma unroll 1
+  for (int e = l; e < N1 * N1; e += 64) {
+    const int s = e / N1, j = e - (e / N1) * N1;
+    A::at(mat, e) = sm[sstep[s] * LD + j];
+  }
+  if (l < N1) mat.perm(MS::PERM, l) = sstep[l];
+  __syncthreads();   // the LDS is reused by the next cell of this block
+  return sing;
+}
+
+template <class M, class Real>
+__device__ __forceinline__ void lu_mfma_one(const ReactArgs& a, const int* __restrict__ list, const long n,
+                                            const long idx) {
+  constexpr int N1 = M::N1;
+  constexpr int NP = MfmaDim<N1>::NP, LD = MfmaDim<N1>::LD;
+  using S = Slots<N1>;
+  constexpr bool IS32 = sizeof(Real) == 4;
+  __shared__ Real sm[NP * LD];
+  __shared__ int sdone[64], sstep[64];
+  long c = -1;
+  bool work = false;
+  double gamma = 0.0;
+  if (idx < n) {
+    c = list[idx];
+    const WS w = cell_ws<M>(a.ws, c);
+    const bool pend = LANE_FIELD(M, w, setup_pending) != 0;
+    const bool fp32_cell = (a.lu_mode == LU_FP32_IR) && LANE_FIELD(M, w, lu64) == 0;
+    work = pend && (fp32_cell == IS32);
+    gamma = LANE_FIELD(M, w, gamma);
+  }
+  const int sing = lu_mfma_core<N1, Real>(cell_mat<M>(a.mat, c < 0 ? 0 : c), work, gamma, sm, sdone, sstep);
+  if (work && threadIdx.x == 0) cell_ws<M>(a.ws, c)[S::SING] = (double)sing;
+}
+
+template <class M, class Real, bool STRIDE>
+__global__ void __launch_bounds__(64) k_lu_mfma(const ReactArgs a, const int* __restrict__ list,
+                                                const unsigned int* __restrict__ count) {
+  const long n = (long)*count;
+  if (STRIDE) {
+    for (long blk = xcd_block(); blk < n; blk += gridDim.x) lu_mfma_one<M, Real>(a, list, n, blk);
+  } else {
+    lu_mfma_one<M, Real>(a, list, n, xcd_block());
+  }
+}
+
 // true when the cell's Newton iteration proceeds this trip (not a singular set-up)
 template <class M>
 __device__ __forceinline__ bool solve_due(const WS& w) {
@@ -2523,6 +2729,22 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
                      bool exact = true) {
   if (a.lu_lane) {
     hipLaunchKernelGGL(k_lu_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, list, count);
+    return hipGetLastError();
+  }
+  if (a.lu_mfma) {   // one cell per block; a multiple of 8 blocks (XCD mapping)
+    const dim3 g(exact ? (unsigned)(n > 0 ? ((n + 7) / 8) * 8 : 8) : 8192u);
+    if (a.lu_mode == LU_FP32_IR) {
+      if (exact)
+        hipLaunchKernelGGL((k_lu_mfma<M, float, false>), g, dim3(64), 0, stream, a, list, count);
+      else
+        hipLaunchKernelGGL((k_lu_mfma<M, float, true>), g, dim3(64), 0, stream, a, list, count);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    if (exact)
+      hipLaunchKernelGGL((k_lu_mfma<M, double, false>), g, dim3(64), 0, stream, a, list, count);
+    else
+      hipLaunchKernelGGL((k_lu_mfma<M, double, true>), g, dim3(64), 0, stream, a, list, count);
     return hipGetLastError();
   }
   const dim3 grid(coop_grid<M::N1>(n, exact));

@@ -46,6 +46,8 @@ struct ReactArgs {
   int ir_sweeps;
   double ir_tol;                         // a refinement sweep whose correction has WRMS norm (in
                                          // units of the error weights) <= ir_tol ends the refinement
+  int lu_mfma;                           // 1: the factorisations on the matrix cores (one cell
+                                         // per wave, blocked; the solves stay cooperative)
   int lu_lane;                           // 1: one lane factors a cell (the form that runs when
                                          // the kernels execute serially on a CPU); 0: 32 lanes
   // per-cell workspace, tiled by 64 cells: element (slot s) of cell c at

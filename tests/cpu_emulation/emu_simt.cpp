@@ -11,6 +11,7 @@
 using namespace pelelm;
 
 namespace {
+int g_lu_mfma = 0;
 struct Host {
   std::vector<double> ws, mat;
   std::vector<int> iws, lists;
@@ -48,6 +49,7 @@ ReactArgs make_args(Host& h, int m, long ncell, long slots, double* rY, double* 
   a.nlist = h.lists.data() + 5 * ls;
   a.counts = h.counts;
   a.lu_lane = 0;
+  a.lu_mfma = g_lu_mfma;
   a.lu_mode = lu_mode;
   a.ir_sweeps = 3;
   a.ir_tol = 1e-4;
@@ -55,6 +57,11 @@ ReactArgs make_args(Host& h, int m, long ncell, long slots, double* rY, double* 
   return a;
 }
 }  // namespace
+
+// 1: the following calls factor on the (emulated) matrix cores (mi.lu_form = mfma)
+extern "C" void simt_set_lu_mfma(int on) { g_lu_mfma = on; }
+// emulated matrix-core products so far (per lane)
+extern "C" long simt_mfma_count() { return simt_mfma_calls; }
 
 // the host-driven trips with the cooperative LU and solves (compact: 0 plain, 1 queue)
 extern "C" int simt_react_trips(const char* mech, long ncell, double* rY, double* T, double* rE, double* FC,

@@ -68,6 +68,9 @@ inline void __syncthreads() {}
 // matrix-core products need all 64 lanes at once: kernels that use them (the learned
 // predictor) are compiled here but never launched
 #define __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, x, y, z) (c)
+// one lane at a time cannot run a matrix-core product (mi.lu_form = mfma is not launched here)
+#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) (c)
+#define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) (c)
 #define __shared__ static
 inline unsigned long long __ballot(int p) { return p ? (1ull << __lane_id()) : 0ull; }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }

@@ -322,6 +322,37 @@ inline int simt_readlane(int v, int l) {
 // the learned predictor's matrix-core products are not emulated (its kernels are not
 // launched here)
 #define __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, x, y, z) (c)
+// v_mfma_{f64,f32}_16x16x4: D = C + A B over one 16 x 16 x 4 block, with the operand
+// layout measured on gfx950: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; its
+// register v holds D[l / 16 + 4 v][l % 16] (FP64) or D[4 (l / 16) + v][l % 16] (FP32).
+// The products are added to C in k order.  Each lane copies the exchanged operands
+// before its next cross-lane operation.
+inline long simt_mfma_calls = 0;   // products run (per lane), for the tests
+template <class T, class V4>
+inline V4 simt_mfma_16x16x4(T a, T b, V4 c) {
+  ++simt_mfma_calls;
+  const int me = simt::W->cur;
+  const int base = me / simt::WAVE * simt::WAVE;
+  T A[64], B[64];
+  const uint64_t* xa = simt::exchange(simt_bits(a), 9);
+  for (int l = 0; l < 64; ++l) {
+    if (!simt::took_part(base + l, me)) simt::die("a matrix-core operation needs all 64 lanes");
+    A[l] = simt_val<T>(xa[base + l]);
+  }
+  const uint64_t* xb = simt::exchange(simt_bits(b), 10);
+  for (int l = 0; l < 64; ++l) B[l] = simt_val<T>(xb[base + l]);
+  const int lane = me - base, j = lane % 16, ig = lane / 16;
+  V4 d = c;
+  for (int v = 0; v < 4; ++v) {
+    const int i = sizeof(T) == 8 ? ig + 4 * v : 4 * ig + v;
+    T acc = c[v];
+    for (int k = 0; k < 4; ++k) acc += A[i + 16 * k] * B[j + 16 * k];
+    d[v] = acc;
+  }
+  return d;
+}
+#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) simt_mfma_16x16x4<double>((a), (b), (c))
+#define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) simt_mfma_16x16x4<float>((a), (b), (c))
 
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline long long __double_as_longlong(double x) { long long r; std::memcpy(&r, &x, 8); return r; }

@@ -337,6 +337,19 @@ __global__ void k_lu_read(double* x, double* sing, double* lu, int* perm, double
   ir[2 * c + 1] = mat[MS::IRF];
 }
 
+#ifndef UNIT_HOST
+// one 16 x 16 x 4 matrix-core product on raw lane registers: lane l passes a[l], b[l]
+// and c[4 l + v] (v = 0..3) and returns its four result registers in d[4 l + v]
+template <class Real>
+__global__ void __launch_bounds__(64) k_mfma_probe(const double* a, const double* b, const double* c, double* d) {
+  const int l = threadIdx.x;
+  typename MatCore<Real>::V4 cv;
+  for (int v = 0; v < 4; ++v) cv[v] = (Real)c[4 * l + v];
+  cv = MatCore<Real>::mac((Real)a[l], (Real)b[l], cv);
+  for (int v = 0; v < 4; ++v) d[4 * l + v] = (double)cv[v];
+}
+#endif
+
 template <class M>
 int unit_lu_t(int coop, int fp32, int sweeps, long n, const double* J, const double* gamma, const int* pend,
               const double* b, const double* ewt, double* x, double* sing, double* lu, int* perm, double* ir) {
@@ -344,6 +357,7 @@ int unit_lu_t(int coop, int fp32, int sweeps, long n, const double* J, const dou
 #ifdef UNIT_HOST
   if (coop) return -2;   // lanes of the cooperative kernels cannot run one at a time
 #endif
+  // coop: 0 one lane per cell, 1 the cooperative LU, 2 the blocked LU on the matrix cores
   const long nw = (n + 63) / 64;
   double *dJ = dev_copy(J, n * N1 * N1), *dg = dev_copy(gamma, n), *db = dev_copy(b, n * N1);
   double* de = dev_copy(ewt, n * N1);
@@ -369,6 +383,7 @@ int unit_lu_t(int coop, int fp32, int sweeps, long n, const double* J, const dou
   a.ir_sweeps = sweeps;
   a.ir_tol = 1e-4;
   a.lu_lane = coop ? 0 : 1;
+  a.lu_mfma = coop == 2 ? 1 : 0;
   hipLaunchKernelGGL(k_lu_prep<M>, dim3((unsigned)nw), dim3(64), 0, nullptr, dJ, dg, dp, db, de, ws, mats, n);
   hcounts[0] = (unsigned)n;
   UNIT_H2D(dcount, hcounts, sizeof(hcounts));
@@ -424,6 +439,26 @@ extern "C" int unit_lu(const char* mech, int coop, int fp32, int sweeps, long n,
   if (m == MECH_LIDRYER)
     return unit_lu_t<mech_LiDryer>(coop, fp32, sweeps, n, J, gamma, pend, b, ewt, x, sing, lu, perm, ir);
   return -1;
+}
+
+// f64: 1 v_mfma_f64_16x16x4_f64, 0 v_mfma_f32_16x16x4_f32 (operands rounded to FP32);
+// a, b: 64 lane values, c, d: 256 (lane-major, four registers per lane)
+extern "C" int unit_mfma_probe(int f64, const double* A, const double* B, const double* Cm, double* D) {
+#ifdef UNIT_HOST
+  (void)f64; (void)A; (void)B; (void)Cm; (void)D;
+  return -2;   // a matrix-core product needs a whole wave
+#else
+  double *dA = dev_copy(A, 64), *dB = dev_copy(B, 64), *dC = dev_copy(Cm, 256), *dD = dev_zero<double>(256);
+  if (f64)
+    hipLaunchKernelGGL(k_mfma_probe<double>, dim3(1), dim3(64), 0, nullptr, dA, dB, dC, dD);
+  else
+    hipLaunchKernelGGL(k_mfma_probe<float>, dim3(1), dim3(64), 0, nullptr, dA, dB, dC, dD);
+  const hipError_t e = hipGetLastError();
+  UNIT_SYNC();
+  UNIT_D2H(D, dD, 256 * sizeof(double));
+  for (double* p : {dA, dB, dC, dD}) UNIT_DEV_FREE(p);
+  return e == hipSuccess ? 0 : -3;
+#endif
 }
 
 extern "C" int unit_rhs(const char* mech, long n, const double* y, const double* t, const double* rE0,

@@ -393,6 +393,30 @@ def test_one_lane_linear_algebra_matches_oracle(wl, mech, lu):
                 y_frac=1.0, dT=1e-3, fc=0.95)
 
 
+@pytest.mark.parametrize("wl,mech", [("drm19_128_mix", "drm19"), ("dodecane53_64_mix", "dodecane53")])
+@pytest.mark.parametrize("lu,graphs", [("fp64", 0), ("fp32_ir", 0), ("fp64", 8)])
+def test_matrix_core_lu_matches_oracle(wl, mech, lu, graphs):
+    """mi.lu_form = mfma: the blocked LU on 16 x 16 x 4 matrix-core tiles in the reactor,
+    FP64 factors (held to the FP64 step-for-step bounds) and FP32 factors with refinement,
+    host-driven and in graph-captured trips."""
+    n = 4096
+    d = cells.generate_workload(wl, ncell=n)
+    ref = O.Mech(mech).react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    r = _mk(mech, ode__rtol=1e-6, ode__atol=1e-10, mi__lu_form="mfma", mi__lu=lu, mi__graph_trips=graphs)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    assert st["nfail"] == 0 and st["ncells"] == n
+    label = f"{wl} mfma {lu} graphs={graphs}"
+    if lu == "fp64":
+        _parity(label, rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)
+    else:
+        _parity(label, rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref, y_frac=1.0, dT=1e-3, fc=0.95)
+
+
 def test_one_lane_solves_in_graph_trips_cover_every_cell():
     """mi.lu_form = lane with graph-captured trips on more cells than one fixed solve grid
     of 16,384 lanes holds: the one-lane solve's grid is sized for every cell, so cells

@@ -342,3 +342,106 @@ def test_step_control_gpu_matches_cpu(mech):
     _dump(f"stepctl_{mech}", og=og, oc=oc, pg=pg, pc=pc)
     np.testing.assert_allclose(pg, pc, rtol=1e-15)
     np.testing.assert_allclose(og, oc, rtol=1e-12, atol=1e-300)
+
+
+def test_matrix_core_probe_needs_a_wave_on_the_cpu_twin():
+    Cl = _load("libunit_cpu.so")
+    z = np.zeros(256)
+    assert Cl.unit_mfma_probe(1, _p(z), _p(z), _p(z), _p(z)) == -2
+
+
+def _mfma_pack(A, B, Cm, f64):
+    """Lane registers of one 16 x 16 x 4 product: lane l holds A[l % 16][l / 16] and
+    B[l / 16][l % 16]; register v of lane l holds D[l / 16 + 4 v][l % 16] in the FP64
+    product and D[4 (l / 16) + v][l % 16] in the FP32 one (the layout the blocked LU uses)."""
+    lane = np.arange(64)
+    a, b = A[lane % 16, lane // 16], B[lane // 16, lane % 16]
+    rows = np.array([[(l // 16 + 4 * v) if f64 else (4 * (l // 16) + v) for v in range(4)] for l in lane])
+    cols = np.repeat((lane % 16)[:, None], 4, axis=1)
+    return np.ascontiguousarray(a), np.ascontiguousarray(b), rows.reshape(-1), cols.reshape(-1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("f64", [1, 0])
+def test_matrix_core_operand_layout(f64):
+    """One v_mfma_{f64,f32}_16x16x4 product on raw lane registers reproduces C + A B with
+    the operand layout the blocked LU assumes; a transposed or shifted operand fails by
+    O(1)."""
+    G = _load("libunit_gpu.so")
+    rng = np.random.default_rng(5 + f64)
+    A, B, Cm = rng.normal(size=(16, 4)), rng.normal(size=(4, 16)), rng.normal(size=(16, 16))
+    if not f64:
+        A, B, Cm = (x.astype(np.float32).astype(np.float64) for x in (A, B, Cm))
+    a, b, rows, cols = _mfma_pack(A, B, Cm, f64)
+    c = np.ascontiguousarray(Cm[rows, cols])
+    d = np.zeros(256)
+    assert G.unit_mfma_probe(f64, _p(a), _p(b), _p(c), _p(d)) == 0
+    D = np.full((16, 16), np.nan)
+    D[rows, cols] = d
+    ref = Cm + A @ B
+    eps = 2.0 ** -52 if f64 else 2.0 ** -23
+    bound = 8 * eps * (np.abs(Cm) + np.abs(A) @ np.abs(B))
+    assert np.all(np.abs(D - ref) <= bound), np.abs(D - ref).max()
+    assert np.abs(D - (Cm + A @ B[:, ::-1])).max() > 1e-3   # the check can fail
+    wrong = np.full((16, 16), np.nan)
+    _, _, rows_other, _ = _mfma_pack(A, B, Cm, 1 - f64)
+    wrong[rows_other, cols] = d
+    assert np.abs(wrong - ref).max() > 1e-3                 # the two row maps differ
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mech,wl", CASES)
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_matrix_core_lu_matches_cooperative_lu(mech, wl, fp32):
+    """The blocked LU (panels of four columns, the trailing update as 16 x 16 x 4 matrix-
+    core tiles) against the cooperative factorisation on reactor Jacobians, FP64 and
+    FP32 factors: the same singular columns reported, valid permutations that agree
+    with the cooperative pivots (a near-tie may break differently where the two round
+    differently), factors that reproduce P M, solves that reach the same solution (with
+    FP32 factors after refinement), and cells with no set-up pending left untouched."""
+    G = _load("libunit_gpu.so")
+    m, N1, y, t, rE0, fH, fy = _inputs(mech, wl, 645)
+    N1, n, J, gamma, pend, b = _lu_case(mech, wl)
+    sing_cells = np.arange(5, n, 97)
+    for i, c in enumerate(sing_cells):
+        k = (3 * i) % N1
+        gamma[c] = 2.0 ** -20
+        J[c, :, k] = 0.0
+        J[c, k, k] = 2.0 ** 20
+        pend[c] = 1
+    M = np.eye(N1)[None] - gamma[:, None, None] * J
+    ewt = None
+    if fp32:
+        xe = np.linalg.solve(M + np.where(np.isin(np.arange(n), sing_cells), 1.0, 0.0)[:, None, None] * np.eye(N1),
+                             b[..., None])[..., 0]
+        ewt = 1.0 / (1e-6 * np.abs(xe) + 1e-10 * np.abs(xe).max(axis=1, keepdims=True))
+    xm, sm, lum, pm, irm = run_lu(G, mech, 2, J, gamma, pend, b, fp32=fp32, ewt=ewt)
+    xc, sc, luc, pc, irc = run_lu(G, mech, 1, J, gamma, pend, b, fp32=fp32, ewt=ewt)
+    _dump(f"lu_mfma_{mech}_{fp32}", xm=xm, xc=xc, sm=sm, sc=sc, pm=pm, pc=pc, lum=lum, luc=luc)
+    idle = pend == 0
+    assert np.all(sm[idle] == -3.0) and np.all(lum[idle] == 7.0) and np.all(pm[idle] == -1)
+    np.testing.assert_array_equal(sm[pend == 1], sc[pend == 1])
+    is_sing = np.zeros(n, bool)
+    is_sing[sing_cells] = True
+    assert np.all(sm[is_sing] > 0)
+    ok = (pend == 1) & (sc == 0)
+    for c in np.where(pend == 1)[0]:
+        assert sorted(pm[c]) == list(range(N1))
+    agree = np.all(pm[ok] == pc[ok], axis=1)
+    assert agree.mean() > (0.97 if fp32 else 0.99), agree.mean()
+    Mo = M[ok]
+    Lf = np.tril(lum[ok], -1) + np.eye(N1)[None]
+    PM = np.take_along_axis(Mo, pm[ok][:, :, None].astype(np.int64), axis=1)
+    res = np.abs(PM - Lf @ np.triu(lum[ok])).max(axis=(1, 2)) / np.abs(Mo).max(axis=(1, 2))
+    assert res.max() < (1e-5 if fp32 else 1e-13), res.max()
+    same = ok.copy()
+    same[ok] = agree
+    scale = np.abs(luc[same]).max(axis=2, keepdims=True)
+    assert np.max(np.abs(lum[same] - luc[same]) / scale) < (1e-5 if fp32 else 1e-13)
+    xs = np.abs(xc[ok]).max(axis=1, keepdims=True)
+    if fp32:
+        conv = (irm[ok, 1] == 0) & (irc[ok, 1] == 0)
+        assert conv.mean() > (0.99 if N1 > 32 else 1.0 - 1e-12)
+        assert np.max(np.abs(xm[ok][conv] - xc[ok][conv]) * ewt[ok][conv]) < 2e-3
+    else:
+        assert np.max(np.abs(xm[ok] - xc[ok]) / xs) < 1e-12

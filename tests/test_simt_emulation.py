@@ -35,11 +35,12 @@ def simt():
     stale = not os.path.exists(LIB) or any(os.path.getmtime(s) > os.path.getmtime(LIB) for s in SRC)
     if stale:
         os.makedirs(os.path.dirname(LIB), exist_ok=True)
-        subprocess.run([CLANG, "-O2", "-std=c++17", "-pthread", "-fPIC", "-shared", "-I" + os.path.join(EMU, "simt"),
+        subprocess.run([CLANG, "-O2", "-std=c++17", "-pthread", "-fPIC", "-shared", "-Wno-psabi", "-I" + os.path.join(EMU, "simt"),
                         "-o", LIB, SRC[0]], check=True)
     L = C.CDLL(LIB)
     L.simt_react_trips.restype = C.c_int
     L.simt_react_persist.restype = C.c_int
+    L.simt_mfma_count.restype = C.c_long
     return L
 
 
@@ -59,8 +60,11 @@ def _check(label, d, ref, rY, T, FC, cnt, n, lu):
 
 
 @pytest.mark.parametrize("wl,mech,n", [("drm19_64", "drm19", 128), ("dodecane53_64_mix", "dodecane53", 64)])
-@pytest.mark.parametrize("compact", [0, 1])
-def test_cooperative_trip_kernels_match_oracle(simt, wl, mech, n, compact):
+@pytest.mark.parametrize("compact,mfma", [(0, 0), (1, 0), (0, 1)])
+def test_cooperative_trip_kernels_match_oracle(simt, wl, mech, n, compact, mfma):
+    """mfma = 1 factors with the blocked LU on (emulated) 16x16x4 matrix-core products."""
+    simt.simt_set_lu_mfma(mfma)
+    before = simt.simt_mfma_count()
     d = cells.generate_workload(wl, ncell=n)
     ref = O.Mech(mech).react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10, max_steps=20000)
     rY, T, rE = d["rY"].copy(), d["T"].copy(), d["rE"].copy()
@@ -69,8 +73,10 @@ def test_cooperative_trip_kernels_match_oracle(simt, wl, mech, n, compact):
     trips = C.c_long(0)
     rc = simt.simt_react_trips(mech.encode(), C.c_long(n), _p(rY), _p(T), _p(rE), _p(FC), C.c_double(d["dt"]),
                                C.c_double(1e-6), C.c_double(1e-10), compact, 0, _p(cnt), C.byref(trips))
+    simt.simt_set_lu_mfma(0)
     assert rc == 0
-    _check(f"{wl} trips compact={compact}", d, ref, rY, T, FC, cnt, n, 0)
+    assert (simt.simt_mfma_count() > before) == bool(mfma)
+    _check(f"{wl} trips compact={compact} mfma={mfma}", d, ref, rY, T, FC, cnt, n, 0)
 
 
 @pytest.mark.parametrize("wl,mech,n,waves,chunk", [("drm19_128_mix", "drm19", 256, 2, 1),
