@@ -27,6 +27,8 @@ schedule[+modifier...]:
             correct the FP32 solves)
   +mfma  -- the blocked LU on the matrix cores (mi.lu_form = mfma: 16x16x4 FP64 or FP32
             tiles for the trailing updates) in place of the cooperative LU
+  +gmres -- cvode.solve_type = GMRES, the reference's GPU decks' solve: no Jacobian, no
+            factorisation, the Newton systems by GMRES(5) with difference-quotient J v
   +graphs -- trips captured into HIP graphs of 8
   +pred  -- the learned Newton predictor, trained before the timed region on pairs the
             plain integrator generates from cells of a seed of their own (setup time
@@ -124,14 +126,19 @@ def kernel_model(st, nspec, mech):
     evals = st["kernel_cells"][0] or nfe
     consume_passes = st["kernel_cells"][2] or nfe
     nir = st.get("nlu_refine", 0)
+    nli = st.get("nli", 0)
     return {
         "k_eval": (nfe * fixed + st["nhy2t_iters"] * per_iter, evals * rhs_bytes(N)),
         # Jacobian where evaluated; M = I - gamma J formation is in the LU kernel
         "k_setup": (nje * jac_flops, nje * 8 * (n * n + 2 * n + 2 * LANE_WORDS)),
         "k_consume": (nni * 8 * n + nst * 2 * 5 * n * 3,
                       consume_passes * 8 * (LANE_WORDS * 2 + 3 * n) + nni * 8 * 3 * n + nst * 8 * 2 * 5 * n),
-        # forward/back substitution, plus per refinement sweep a residual and a second solve
-        "k_solve_coop": (nni * 2 * n * n + nir * 4 * n * n, nni * 8 * (n * n + 2 * n)),
+        # forward/back substitution, plus per refinement sweep a residual and a second solve;
+        # with GMRES per Krylov iteration one right-hand side (its J v product), the
+        # modified Gram-Schmidt pass (4 n per basis vector, 3 vectors on average) and the
+        # vector updates, reading and writing the basis in the workspace
+        "k_solve_coop": ((nni * 2 * n * n + nir * 4 * n * n) if not nli else nli * (fixed + 20 * n),
+                         nni * 8 * (n * n + 2 * n) if not nli else nli * 8 * (12 * n + N + LANE_WORDS)),
         # I - gamma J (n^2) and the factorisation
         "k_lu_coop": (nsetups * (2 * n ** 3 // 3 + 2 * n * n + n * n), nsetups * 8 * 2 * n * n),
         "k_predict": (st["npred_used"] * 2 * ((N + 3) * 128 + 2 * 128 * 128 + 128 * n), st["npred_used"] * 8 * 3 * n),
@@ -225,6 +232,8 @@ def run_form(form, mech, dev_in, ncell, nspec, dt, steps, warmup, profile, net=N
         r.set("mi.ir_sweeps", 0)
     if "mfma" in mods:
         r.set("mi.lu_form", "mfma")
+    if "gmres" in mods:
+        r.set("cvode.solve_type", "GMRES")
     r.init(2, ncell)
     if "pred" in mods:
         predictor.attach(r, net)

@@ -81,6 +81,8 @@ typedef struct {
   long schedule_used;
   long graph_trips_used;
   double tune_ms[4];
+  long nli;             /* GMRES iterations (cvode.solve_type = GMRES); each evaluates the
+                           right-hand side once for its J v product, not counted in nfe */
 } mi_chem_stats;
 
 /* ReactorBase::create(name) -- Source/PeleLM.cpp:706.
@@ -102,7 +104,10 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
 /* Runtime options, the ParmParse keys the reference decks use
    (Exec/RegTests/FlameSheet/inputs.3d-regt:84-90) plus this reactor's own:
      ode.rtol, ode.atol, cvode.max_order, cvode.max_num_steps (default 20000),
-     cvode.solve_type (denseAJ_direct), peleLM.use_typ_vals_chem (0/1),
+     cvode.solve_type (denseAJ_direct: LU of I - gamma J with the analytic Jacobian, default
+     | dense_direct: the same | GMRES: matrix-free scaled GMRES, J v by a difference quotient
+     of the right-hand side, no set-up; the trip schedules only, mi.gmres_maxl Krylov
+     vectors, default 5 as CVODE's, at most the number of species), peleLM.use_typ_vals_chem (0/1),
      mi.mechanism (drm19 | LiDryer | dodecane53), mi.schedule (plain: fixed cell-to-lane mapping
      for the evaluation and step kernels, default | queue: finished cells leave the work
      lists | auto: the first call of each box size searches the plain and queue schedules

@@ -15,7 +15,9 @@
  *     (maxcor 3), dense partial-pivot LU of I - gamma J, WRMS error control and
  *     CVODE's step/order heuristics (the solver the reference decks select with
  *     cvode.solve_type = denseAJ_direct, cvode.max_order = 4,
- *     Exec/RegTests/FlameSheet/inputs.3d-regt:84-90).  SUNDIALS is not vendored in the
+ *     Exec/RegTests/FlameSheet/inputs.3d-regt:84-90), or scaled GMRES with difference-
+ *     quotient J v products (cvode.solve_type = GMRES, the GPU decks,
+ *     Exec/RegTests/FlameSheet/inputs.3d-regt_GPU:89).  SUNDIALS is not vendored in the
  *     reference (Submodules/sundials is empty), so this restates CVODE's published
  *     algorithm (Hindmarsh et al., ACM TOMS 31(3), 2005; CVODE user guide, Mathematical
  *     Considerations).
@@ -75,6 +77,8 @@ typedef struct {
   const double* atol_vec;    /* ... or per-component (N+1) values, nullable          */
   int max_order;             /* 1..5 */
   long max_steps;
+  int solve_type;            /* 0 dense direct LU, 1 GMRES (matrix-free, spgmr.hpp) */
+  int gmres_maxl;            /* Krylov dimension of the GMRES solves, 0: CVODE's default 5 */
 } orc_opts;
 
 typedef struct {
@@ -82,6 +86,7 @@ typedef struct {
   int status;                /* 0 ok, <0 failure code */
   int qlast;
   double hlast;
+  long nli;                  /* GMRES iterations (solve_type 1) */
 } orc_cellstats;
 
 /* Integrate every cell with mask >= 0 from 0 to dt (h-reactor, constant p).
@@ -113,6 +118,10 @@ void orc_rk64_tableau(double* out30);
 
 /* Radau IIA-5 coefficients as radau5.cpp derives them: c[3], A[3][3], e[3], g0. */
 void orc_radau_coefficients(double* out16);
+
+/* The GMRES core (spgmr.hpp) on a dense A: out[0] residual estimate, out[1] iterations. */
+int orc_spgmr_dense(int n, const double* A, const double* b, const double* w, int maxl, double delta, double* x,
+                    double* out);
 
 /* Integrate the linear test system y' = A y (n x n, row-major) with the same BDF
    machinery (pins the integrator against exp(A t)).  Returns status. */

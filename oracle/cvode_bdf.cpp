@@ -9,13 +9,17 @@
 // Jacobian re-evaluation every 51 steps or after a convergence failure), dense
 // partial-pivot LU of M = I - gamma J, WRMS local error test with BIAS1/2/3 order
 // selection, the initial step estimate from a y'' norm, and CV_NORMAL output
-// (step past tout, interpolate the Nordsieck array back).
+// (step past tout, interpolate the Nordsieck array back).  With solve_type = 1 the
+// Newton systems are solved by scaled GMRES instead (spgmr.hpp; the reference's GPU decks,
+// Exec/RegTests/FlameSheet/inputs.3d-regt_GPU:89): CVODE's matrix-free iterative
+// linear-solver interface, J v by a difference quotient of the right-hand side.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
 
 #include "oracle_internal.hpp"
+#include "spgmr.hpp"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -63,8 +67,52 @@ struct Bdf {
   int q = 1, qprime = 1, L = 2, qwait = 2;
   long nst = 0, nfe = 0, nje = 0, nsetups = 0, nni = 0, ncfn = 0, netf = 0, nstlp = 0, nstlj = 0;
   bool jcur = false;
+  // solve_type 1: GMRES (no set-up, no matrix); nli Krylov iterations, nfe_ls their
+  // right-hand-side evaluations (not counted in nfe, as CVODE counts them apart)
+  int gmres = 0, maxl = 5;
+  long nli = 0, nfe_ls = 0;
 
   void F(double t, const double* yy, double* yd) { ++nfe; P->f(t, yy, yd); }
+
+  // CVODE's iterative linear solve of M x = b, M = I - gamma J (cvLsSolve with
+  // SUNLinSol_SPGMR): a right-hand side already below eplifac tq[4] in the WRMS norm
+  // gives x = b on the first Newton iteration and x = 0 on later ones; otherwise GMRES
+  // in the error-weight scaling to 2-norm tolerance eplifac tq[4] sqrt(n), with
+  // J v = (f(tn, y + sig v) - f(tn, y)) / sig, sig = 1 / ||v||_WRMS, at the current
+  // iterate y (ftemp = f(tn, y)).  A reduced but unconverged residual is accepted on the
+  // first Newton iteration only.  Returns 0, or 1 (recoverable: the Newton iteration
+  // fails).  b is overwritten by x.
+  int gmres_solve(double* b, int mnewt) {
+    constexpr double EPLIFAC = 0.05;
+    const double deltar = EPLIFAC * tq[4];
+    if (wrms(b) <= deltar) {
+      if (mnewt > 0)
+        for (int i = 0; i < n; ++i) b[i] = 0.0;
+      return 0;
+    }
+    const double delta = deltar * std::sqrt((double)n);
+    double work[ORC_NMAX], fw[ORC_NMAX], x[ORC_NMAX];
+    auto atimes = [&](const double* v, double* z) {
+      const double sig = 1.0 / wrms(v);
+      for (int i = 0; i < n; ++i) work[i] = sig * v[i] + y[i];
+      P->f(tn, work, fw);
+      ++nfe_ls;
+      const double siginv = 1.0 / sig;
+      for (int i = 0; i < n; ++i) {
+        const double jv = siginv * fw[i] - siginv * ftemp[i];
+        z[i] = v[i] - gamma * jv;
+      }
+    };
+    double rho = 0.0;
+    int it = 0;
+    const int r = spgmr(n, maxl, b, ewt, delta, atimes, x, &rho, &it);
+    nli += it;
+    if (r == SPGMR_SUCCESS || (r == SPGMR_RES_REDUCED && mnewt == 0)) {
+      for (int i = 0; i < n; ++i) b[i] = x[i];
+      return 0;
+    }
+    return 1;
+  }
 
   void ewtset(const double* yy) {
     for (int i = 0; i < n; ++i) ewt[i] = 1.0 / (rtol * std::fabs(yy[i]) + atol[i]);
@@ -302,6 +350,10 @@ struct Bdf {
     int convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
     bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (nst == 0) ||
                      (nst >= nstlp + MSBP) || (std::fabs(gamrat - 1.0) > DGMAX);
+    if (gmres) {   // no set-up function: no matrix to refresh, the rate estimate restarts
+      callSetup = false;
+      crate = 1.0;
+    }
     for (;;) {
       for (int i = 0; i < n; ++i) { acor[i] = 0.0; y[i] = zn[0][i]; }
       F(tn, y, ftemp);
@@ -319,8 +371,12 @@ struct Bdf {
       for (;;) {
         ++nni;
         for (int i = 0; i < n; ++i) tempv[i] = gamma * ftemp[i] - rl1 * zn[1][i] - acor[i];
-        lu_solve(tempv);
-        if (gamrat != 1.0) {
+        if (gmres) {
+          if (gmres_solve(tempv, m) != 0) { ret = 1; break; }
+        } else {
+          lu_solve(tempv);
+        }
+        if (!gmres && gamrat != 1.0) {
           const double s = 2.0 / (1.0 + gamrat);
           for (int i = 0; i < n; ++i) tempv[i] *= s;
         }
@@ -341,7 +397,7 @@ struct Bdf {
         for (int i = 0; i < n; ++i) y[i] = zn[0][i] + acor[i];
         F(tn, y, ftemp);
       }
-      if (ret > 0 && !jcur) {
+      if (ret > 0 && !jcur && !gmres) {
         callSetup = true;
         convfail = FAIL_BAD_J;
         continue;
@@ -509,6 +565,7 @@ struct Bdf {
     tn = 0.0;
     q = 1; L = 2; qwait = L; etamax = ETAMX1;
     nst = nfe = nje = nsetups = nni = ncfn = netf = 0;
+    nli = nfe_ls = 0;
     nstlp = nstlj = 0;
     saved_tq5 = 0.0;
     crate = 1.0;
@@ -581,6 +638,7 @@ void fill_stats(const Bdf& b, int status, orc_cellstats* st) {
   st->nst = b.nst; st->nfe = b.nfe; st->nje = b.nje; st->nsetups = b.nsetups;
   st->nni = b.nni; st->ncfn = b.ncfn; st->netf = b.netf;
   st->status = status; st->qlast = b.q; st->hlast = b.h;
+  st->nli = b.nli;
 }
 
 }  // namespace
@@ -597,6 +655,8 @@ extern "C" int orc_bdf_linear(int n, const double* A, double* y, double tout, co
   for (int i = 0; i < n; ++i) b->atol[i] = o->atol_vec ? o->atol_vec[i] : o->atol;
   b->qmax = o->max_order;
   b->mxstep = o->max_steps;
+  b->gmres = o->solve_type == 1;
+  b->maxl = o->gmres_maxl > 0 ? o->gmres_maxl : 5;
   double yout[ORC_NMAX];
   const int r = b->integrate(y, tout, yout);
   if (r == 0) for (int i = 0; i < n; ++i) y[i] = yout[i];
@@ -624,6 +684,8 @@ extern "C" int orc_react(const orc_mech* m, int ncell, double* rY, const double*
     for (int i = 0; i < n; ++i) b->atol[i] = o->atol_vec ? o->atol_vec[i] : o->atol;
     b->qmax = o->max_order;
     b->mxstep = o->max_steps;
+    b->gmres = o->solve_type == 1;
+    b->maxl = o->gmres_maxl > 0 ? o->gmres_maxl : 5;
     double y0[ORC_NMAX], yout[ORC_NMAX], fY[ORC_NMAX];
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 4)
@@ -657,4 +719,24 @@ extern "C" int orc_react(const orc_mech* m, int ncell, double* rY, const double*
     delete b;
   }
   return nfail;
+}
+
+/* The GMRES core on a dense matrix (row-major n x n A): solves A x = b in the scaling w
+   to 2-norm tolerance delta with at most maxl Krylov vectors.  out[0] = final residual
+   estimate, out[1] = Krylov iterations.  Returns the SPGMR status (0 converged,
+   1 residual reduced, 2 no reduction, 3 QR breakdown). */
+extern "C" int orc_spgmr_dense(int n, const double* A, const double* b, const double* w, int maxl, double delta,
+                               double* x, double* out) {
+  auto atimes = [&](const double* v, double* z) {
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += A[i * n + j] * v[j];
+      z[i] = s;
+    }
+  };
+  double rho = 0.0;
+  int it = 0;
+  const int r = spgmr(n, maxl, b, w, delta, atimes, x, &rho, &it);
+  if (out) { out[0] = rho; out[1] = it; }
+  return r;
 }

@@ -22,13 +22,14 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 class Opts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("atol_vec", C.c_void_p),
-                ("max_order", C.c_int), ("max_steps", C.c_long)]
+                ("max_order", C.c_int), ("max_steps", C.c_long), ("solve_type", C.c_int),
+                ("gmres_maxl", C.c_int)]
 
 
 class CellStats(C.Structure):
     _fields_ = [("nst", C.c_long), ("nfe", C.c_long), ("nje", C.c_long), ("nsetups", C.c_long),
                 ("nni", C.c_long), ("ncfn", C.c_long), ("netf", C.c_long), ("status", C.c_int),
-                ("qlast", C.c_int), ("hlast", C.c_double)]
+                ("qlast", C.c_int), ("hlast", C.c_double), ("nli", C.c_long)]
 
 
 def build(force: bool = False) -> str:
@@ -84,12 +85,17 @@ def lib():
                                       C.c_int, C.c_void_p]
         L.orc_rk64_linear.restype = C.c_int
         L.orc_rk64_tableau.argtypes = [_dp]
+        L.orc_spgmr_dense.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, C.c_double, _dp, _dp]
+        L.orc_spgmr_dense.restype = C.c_int
         _lib = L
     return _lib
 
 
-def make_opts(rtol=1e-6, atol=1e-10, atol_vec=None, max_order=4, max_steps=100000):
-    o = Opts(rtol, atol, None, max_order, max_steps)
+SOLVE_TYPES = {"dense": 0, "gmres": 1}
+
+
+def make_opts(rtol=1e-6, atol=1e-10, atol_vec=None, max_order=4, max_steps=100000, solve="dense", maxl=0):
+    o = Opts(rtol, atol, None, max_order, max_steps, SOLVE_TYPES[solve], int(maxl))
     keep = None
     if atol_vec is not None:
         keep = np.ascontiguousarray(atol_vec, dtype=np.float64)
@@ -166,8 +172,10 @@ class Mech:
 
     # -- reactor ------------------------------------------------------------
     def react(self, rY, T, rE, dt, rYsrc=None, rEsrc=None, mask=None, rtol=1e-6, atol=1e-10,
-              atol_vec=None, max_order=4, max_steps=100000, nthreads=0, stats=False):
-        """Advance SoA cell states (rY: [nspec, ncell]) in place-copy; returns new arrays."""
+              atol_vec=None, max_order=4, max_steps=100000, nthreads=0, stats=False, solve="dense", maxl=0):
+        """Advance SoA cell states (rY: [nspec, ncell]) in place-copy; returns new arrays.
+        solve: "dense" (LU of I - gamma J) or "gmres" (cvode.solve_type = GMRES, Krylov
+        dimension maxl, 0: CVODE's default 5)."""
         rY = np.array(rY, dtype=np.float64, order="C", copy=True)
         T = np.array(T, dtype=np.float64, copy=True)
         rE = np.array(rE, dtype=np.float64, copy=True)
@@ -176,7 +184,7 @@ class Mech:
         src = None if rYsrc is None else np.ascontiguousarray(rYsrc, dtype=np.float64)
         esrc = None if rEsrc is None else np.ascontiguousarray(rEsrc, dtype=np.float64)
         msk = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
-        o, keep = make_opts(rtol, atol, atol_vec, max_order, max_steps)
+        o, keep = make_opts(rtol, atol, atol_vec, max_order, max_steps, solve, maxl)
         st = (CellStats * ncell)() if stats else None
         nfail = lib().orc_react(self._h, ncell, rY, None if src is None else src.ctypes.data, T, rE,
                                 None if esrc is None else esrc.ctypes.data, FC,
@@ -240,10 +248,21 @@ def radau_coefficients():
     return out[:3], out[3:12].reshape(3, 3), out[12:15], out[15]
 
 
-def bdf_linear(A, y0, tout, rtol=1e-8, atol=1e-12, max_order=5):
+def spgmr_dense(A, b, w, maxl, delta):
+    """The GMRES core on a dense matrix: (status, x, residual estimate, iterations)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    x = np.zeros(A.shape[0])
+    out = np.zeros(2)
+    r = lib().orc_spgmr_dense(A.shape[0], A.ravel(), b, w, int(maxl), float(delta), x, out)
+    return r, x, out[0], int(out[1])
+
+
+def bdf_linear(A, y0, tout, rtol=1e-8, atol=1e-12, max_order=5, solve="dense", maxl=0):
     A = np.ascontiguousarray(A, dtype=np.float64)
     y = np.array(y0, dtype=np.float64, copy=True)
-    o, _ = make_opts(rtol, atol, None, max_order, 100000)
+    o, _ = make_opts(rtol, atol, None, max_order, 100000, solve, maxl)
     st = CellStats()
     r = lib().orc_bdf_linear(A.shape[0], A.ravel(), y, float(tout), C.byref(o), C.byref(st))
     return y, r, st

@@ -57,6 +57,9 @@ struct Reactor {
   long max_trips = 0;           // 0: 3 x cvode.max_num_steps + 100 trips
   int lu_lane = 0;
   int lu_mfma = 0;
+  // cvode.solve_type = GMRES: the Newton systems by matrix-free scaled GMRES, Krylov
+  // dimension mi.gmres_maxl (CVODE's default 5)
+  int gmres = 0, gmres_maxl = 5;
   void* predictor = nullptr;    // attached network (mi_chem_attach_predictor), not owned
   int graph_trips = 0;          // 0: host-driven trips; K (even): graphs of K trips
   // mi.schedule = persistent: waves (0: as many as the device holds resident), refill
@@ -295,7 +298,7 @@ int react_persist(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_
 // kept per box size and, with mi.tune_cache = <file>, in a JSON file keyed by mechanism,
 // GPU, tolerances and box size, so later processes skip the search.
 // ---------------------------------------------------------------------------
-std::vector<TuneConfig> tune_candidates() {
+std::vector<TuneConfig> tune_candidates(bool with_persistent) {
   std::vector<TuneConfig> c;
   for (int sched : {SCHED_PLAIN, SCHED_COMPACT})
     for (int g : {0, 8}) {
@@ -308,6 +311,7 @@ std::vector<TuneConfig> tune_candidates() {
   TuneConfig u;
   u.order = ORDER_NONE;
   c.push_back(u);
+  if (!with_persistent) return c;
   for (int chunk : {1, 8, 32}) {
     TuneConfig t;
     t.schedule = SCHED_PERSIST;
@@ -478,7 +482,7 @@ int autotune(Reactor* r, const ReactArgs& a0, hipStream_t stream, long max_trips
   } fr{{&rY, &T, &rE, &FC, &cY, &cT, &cE, &fY, &fE}};
   for (double** p : {&rY, &cY, &fY}) HIPCHK(r, hipMalloc(p, sizeof(double) * N * n));
   for (double** p : {&T, &rE, &FC, &cT, &cE, &fE}) HIPCHK(r, hipMalloc(p, sizeof(double) * n));
-  std::vector<TuneConfig> cand = tune_candidates();
+  std::vector<TuneConfig> cand = tune_candidates(!r->gmres);
   const long strides[3] = {16, 4, 1};
   const size_t keep[3] = {3, 2, 1};
   std::vector<float> last_ms;
@@ -552,6 +556,18 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   a.ir_tol = r->ir_tol;
   a.lu_lane = r->lu_lane;
   a.lu_mfma = r->lu_mfma;
+  a.gmres = (r->gmres && !r->rk64 && !r->null_reactor) ? 1 : 0;
+  a.gmres_maxl = r->gmres_maxl;
+  if (a.gmres) {
+    if (r->gmres_maxl > N1 - 1)
+      return fail(r, MI_EINVAL, "mi.gmres_maxl must not exceed the number of species (the Krylov vectors share "
+                                "the Jacobian's workspace)");
+    if (fixed || r->schedule == SCHED_PERSIST)
+      return fail(r, MI_EINVAL, "cvode.solve_type = GMRES runs on the trip schedules (plain, queue, auto), not "
+                                "the persistent kernel");
+    if (r->predictor) return fail(r, MI_EINVAL, "the learned predictor needs the direct solve");
+    a.lu_mode = LU_FP64;   // no factorisation
+  }
   a.compact = (r->schedule == SCHED_COMPACT) ? 1 : 0;
   int graph_trips = r->graph_trips;
   a.cell_status = nullptr;
@@ -838,8 +854,12 @@ int mi_chem_set(void* reactor, const char* key, const char* value) {
     if (!parse_long(value, &l) || l < 1) return fail(r, MI_EINVAL, "cvode.max_num_steps must be >= 1");
     r->max_steps = l;
   } else if (k == "cvode.solve_type") {
-    if (v != "denseAJ_direct" && v != "dense_direct")
-      return fail(r, MI_EINVAL, "cvode.solve_type: only the dense direct solve is implemented");
+    if (v == "denseAJ_direct" || v == "dense_direct") r->gmres = 0;
+    else if (v == "GMRES") r->gmres = 1;
+    else return fail(r, MI_EINVAL, "cvode.solve_type must be denseAJ_direct, dense_direct or GMRES");
+  } else if (k == "mi.gmres_maxl") {
+    if (!parse_long(value, &l) || l < 1 || l > 64) return fail(r, MI_EINVAL, "mi.gmres_maxl must be 1..64");
+    r->gmres_maxl = (int)l;
   } else if (k == "peleLM.use_typ_vals_chem") {
     if (!parse_long(value, &l) || (l != 0 && l != 1)) return fail(r, MI_EINVAL, "use_typ_vals_chem must be 0 or 1");
     r->use_typ = (l == 1);
@@ -1157,6 +1177,7 @@ int mi_chem_get_stats(void* reactor, mi_chem_stats* out) {
   out->nhy2t_iters = (long)c[CNT_HY2T];
   out->npred_used = (long)c[CNT_PRED];
   out->npred_fallback = (long)c[CNT_PREDFB];
+  out->nli = (long)c[CNT_NLI];
   float ms = 0.f;
   if (r->ev0 && r->ev1) HIPCHK(r, hipEventElapsedTime(&ms, r->ev0, r->ev1));
   out->ms_kernel = ms;

@@ -383,6 +383,7 @@ struct Lane {
   long mxstep;
   double tout;
   int lu_mode, ir_sweeps, use_pred;
+  int gmres;   // the Newton systems by GMRES: no set-up, no matrix
   // the converged correction's norm and the error test's measure, used in the trip that
   // computes them
   double acnrm, dsm;
@@ -582,6 +583,10 @@ struct Lane {
     convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
     callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (nst == 0) ||
                 (nst >= nstlp + MSBP) || (fabs(gamrat - 1.0) > DGMAX);
+    if (gmres) {   // no set-up function: nothing to refresh, the rate estimate restarts
+      callSetup = false;
+      crate = 1.0;
+    }
     m = 0;
     a1 = 0.0; a2 = 0.0; teval = tn;
     phase = PH_NEWTON;
@@ -688,7 +693,7 @@ struct Lane {
       a1 = 0.0; a2 = 0.0; teval = tn;
       return;
     }
-    if (!jcur) {
+    if (!jcur && !gmres) {
       callSetup = true;
       convfail = FAIL_BAD_J;
       m = 0;
@@ -863,7 +868,14 @@ struct Lane {
     double x[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) x[i] = xs[i];
-    if (lu_mode == LU_FP32_IR && !lu64) {
+    if (gmres) {
+      // the linear solve failed (no residual reduction, or a reduction without
+      // convergence after the first Newton iteration): the Newton iteration fails
+      if (mat[MS::IRF] != 0.0) {
+        nls_fail();
+        return false;
+      }
+    } else if (lu_mode == LU_FP32_IR && !lu64) {
       nir += (long)mat[MS::IRS];
       if (mat[MS::IRF] != 0.0) {
         // the FP32 factors cannot deliver this solve: refactor in FP64 and restart the
@@ -879,7 +891,7 @@ struct Lane {
         return false;
       }
     }
-    if (gamrat != 1.0) {
+    if (!gmres && gamrat != 1.0) {   // matrix-free solves use the current gamma
       const double sc = 2.0 / (1.0 + gamrat);
 #pragma unroll
       for (int i = 0; i < N1; ++i) x[i] *= sc;
@@ -1002,6 +1014,7 @@ __device__ __forceinline__ void load_lane(Lane<M>& st, const ReactArgs& a, long 
   st.lu_mode = a.lu_mode;
   st.ir_sweeps = a.ir_sweeps;
   st.use_pred = a.use_pred;
+  st.gmres = a.gmres;
   st.rtol = a.rtol;
   st.mxstep = a.mxstep;
   st.tout = a.dt;
@@ -1117,6 +1130,7 @@ __global__ void __launch_bounds__(64) k_init(const ReactArgs a) {
     st.lu_mode = a.lu_mode;
     st.ir_sweeps = a.ir_sweeps;
     st.use_pred = a.use_pred;
+    st.gmres = a.gmres;
     st.rtol = a.rtol;
     st.mxstep = a.mxstep;
     st.tout = a.dt;
@@ -1948,6 +1962,219 @@ __global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int*
   }
 }
 
+// ---------------------------------------------------------------------------
+// cvode.solve_type = GMRES: the Newton system M x = b, M = I - gamma J, by scaled GMRES
+// with no preconditioner and no matrix (CVODE's iterative linear-solver interface with
+// SUNLinSol_SPGMR, as the reference's GPU decks select it).  One lane per cell.  A
+// right-hand side already below eplifac tq[4] in the WRMS norm gives x = b on the first
+// Newton iteration and x = 0 later.  Otherwise the Krylov space of S M S^-1 on S b,
+// S = diag(error weights), grows to at most maxl vectors (modified Gram-Schmidt with one
+// reorthogonalisation pass when the new vector's norm drops below 1e-3 of its norm before
+// orthogonalisation; Givens rotations reduce the Hessenberg matrix as it grows) until
+// the residual estimate |prod s_i| beta reaches eplifac tq[4] sqrt(n).  Each product
+// M v = v - gamma (f(tn, y + sig v) - f(tn, y)) / sig, sig = 1 / ||v||_WRMS, evaluates
+// the right-hand side (T by HY2T) at the trip's iterate y, whose f the evaluation kernel
+// left in the workspace.  A residual that fell but did not converge is accepted on the
+// first Newton iteration only.  Returns 1 when the Newton iteration must fail.
+// Storage: the Krylov vectors in the workspace's Jacobian slots (unused without a
+// matrix), the Hessenberg matrix in the matrix region's factor block, the rotations in
+// its refinement vectors R and D, the small solution in the TMP slots.
+// ---------------------------------------------------------------------------
+constexpr double GMRES_EPLIFAC = 0.05, GMRES_GS_FACTOR = 1000.0;
+
+template <class M>
+__device__ __forceinline__ int gmres_cell(const WS& w, const CM& mat, const int maxl, const double gamma,
+                                          const double tq4, const int mnewt, const double t, const double rE0,
+                                          const double fH, int& nli) {
+  constexpr int N1 = M::N1;
+  using S = Slots<N1>;
+  using MS = MatSlots<N1>;
+  nli = 0;
+  const double deltar = GMRES_EPLIFAC * tq4;
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < N1; ++i) {
+    const double p = mat[MS::B + i] * w[S::EWT + i];
+    s += p * p;
+  }
+  if (sqrt(s / N1) <= deltar) {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) mat[MS::X + i] = (mnewt > 0) ? 0.0 : mat[MS::B + i];
+    return 0;
+  }
+  const double delta = deltar * sqrt((double)N1);
+  double beta2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < N1; ++i) {
+    const double v = w[S::EWT + i] * mat[MS::B + i];
+    w[S::JS + i] = v;
+    beta2 += v * v;
+  }
+  const double beta = sqrt(beta2);
+  if (beta <= delta) {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) mat[MS::X + i] = 0.0;
+    return 0;
+  }
+  const double rb = 1.0 / beta;
+#pragma unroll
+  for (int i = 0; i < N1; ++i) w[S::JS + i] *= rb;
+  double rho = beta, rotp = 1.0;
+  int krydim = 0;
+  bool converged = false;
+#pragma unroll 1
+  for (int l = 0; l < maxl; ++l) {
+    ++nli;
+    krydim = l + 1;
+    const int vl = S::JS + l * N1, vk = vl + N1;
+    // V[l+1] = S M S^-1 V[l]
+    double vt[N1], work[N1], fw[N1], Tw;
+    double sv = 0.0;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      vt[i] = w[vl + i] / w[S::EWT + i];
+      const double p = vt[i] * w[S::EWT + i];
+      sv += p * p;
+    }
+    const double sig = 1.0 / sqrt(sv / N1);
+#pragma unroll
+    for (int i = 0; i < N1; ++i) work[i] = sig * vt[i] + w[S::Y + i];
+    rhs<M>(work, t, w, rE0, fH, fw, Tw);
+    const double siginv = 1.0 / sig;
+    double vn2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      const double jv = siginv * fw[i] - siginv * w[S::F + i];
+      const double z = w[S::EWT + i] * (vt[i] - gamma * jv);
+      w[vk + i] = z;
+      vn2 += z * z;
+    }
+    // modified Gram-Schmidt against V[0..l]
+    const double vk_norm = sqrt(vn2);
+#pragma unroll 1
+    for (int j = 0; j <= l; ++j) {
+      const int vj = S::JS + j * N1;
+      double h = 0.0;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) h += w[vj + i] * w[vk + i];
+      mat[MS::LU + j * N1 + l] = h;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) w[vk + i] = w[vk + i] - h * w[vj + i];
+    }
+    double nrm2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) nrm2 += w[vk + i] * w[vk + i];
+    double nrm = sqrt(nrm2);
+    const double big = GMRES_GS_FACTOR * vk_norm;
+    if (big + nrm == big) {
+      double newp = 0.0;
+#pragma unroll 1
+      for (int j = 0; j <= l; ++j) {
+        const int vj = S::JS + j * N1;
+        double tdot = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) tdot += w[vj + i] * w[vk + i];
+        newp += tdot * tdot;
+        mat[MS::LU + j * N1 + l] += tdot;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) w[vk + i] = w[vk + i] - tdot * w[vj + i];
+      }
+      if (newp != 0.0) nrm = sqrt(nrm * nrm - newp);
+    }
+    mat[MS::LU + (l + 1) * N1 + l] = nrm;
+    // the earlier rotations on column l, then the rotation that zeroes H[l+1][l]
+#pragma unroll 1
+    for (int i = 0; i < l; ++i) {
+      const double c = mat[MS::R + 2 * i], sn = mat[MS::R + 2 * i + 1];
+      const double t1 = mat[MS::LU + i * N1 + l], t2 = mat[MS::LU + (i + 1) * N1 + l];
+      mat[MS::LU + i * N1 + l] = c * t1 - sn * t2;
+      mat[MS::LU + (i + 1) * N1 + l] = sn * t1 + c * t2;
+    }
+    const double t1 = mat[MS::LU + l * N1 + l], t2 = mat[MS::LU + (l + 1) * N1 + l];
+    double c, sn;
+    if (t2 == 0.0) {
+      c = 1.0;
+      sn = 0.0;
+    } else if (fabs(t2) >= fabs(t1)) {
+      const double t3 = t1 / t2;
+      sn = -1.0 / sqrt(1.0 + t3 * t3);
+      c = -sn * t3;
+    } else {
+      const double t3 = t2 / t1;
+      c = 1.0 / sqrt(1.0 + t3 * t3);
+      sn = -c * t3;
+    }
+    mat[MS::R + 2 * l] = c;
+    mat[MS::R + 2 * l + 1] = sn;
+    const double hll = c * t1 - sn * t2;
+    mat[MS::LU + l * N1 + l] = hll;
+    if (hll == 0.0) return 1;   // breakdown of the Givens-reduced Hessenberg matrix
+    rotp *= sn;
+    rho = fabs(rotp * beta);
+    if (rho <= delta) {
+      converged = true;
+      break;
+    }
+    const double rinv = 1.0 / nrm;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) w[vk + i] *= rinv;
+  }
+  if (!converged && !(rho < beta)) return 1;
+  if (!converged && mnewt > 0) return 1;
+  // y = R^-1 Q^T (beta e1) in the TMP slots, x = S^-1 V y
+  w[S::TMP] = beta;
+#pragma unroll 1
+  for (int i = 1; i <= krydim; ++i) w[S::TMP + i] = 0.0;
+#pragma unroll 1
+  for (int k = 0; k < krydim; ++k) {
+    const double c = mat[MS::R + 2 * k], sn = mat[MS::R + 2 * k + 1];
+    const double a1 = w[S::TMP + k], a2 = w[S::TMP + k + 1];
+    w[S::TMP + k] = c * a1 - sn * a2;
+    w[S::TMP + k + 1] = sn * a1 + c * a2;
+  }
+#pragma unroll 1
+  for (int k = krydim - 1; k >= 0; --k) {
+    const double hkk = mat[MS::LU + k * N1 + k];
+    if (hkk == 0.0) return 1;
+    const double yk = w[S::TMP + k] / hkk;
+    w[S::TMP + k] = yk;
+#pragma unroll 1
+    for (int i = 0; i < k; ++i) w[S::TMP + i] -= yk * mat[MS::LU + i * N1 + k];
+  }
+#pragma unroll
+  for (int i = 0; i < N1; ++i) {
+    double x = w[S::TMP] * w[S::JS + i];
+#pragma unroll 1
+    for (int k = 1; k < krydim; ++k) x += w[S::TMP + k] * w[S::JS + k * N1 + i];
+    mat[MS::X + i] = x / w[S::EWT + i];
+  }
+  return 0;
+}
+
+// the GMRES solves of this trip's Newton systems (cells in the solve list); the flag
+// (MatSlots IRF) tells the step kernel whether the linear solve failed, IRS holds its
+// Krylov iterations
+template <class M>
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_EVAL_WAVES(M::N1)) k_solve_gmres(
+    const ReactArgs a, const int* __restrict__ list, const unsigned int* __restrict__ count) {
+  using MS = MatSlots<M::N1>;
+  const long idx = (long)blockIdx.x * 64 + threadIdx.x;
+  unsigned long long nli = 0;
+  if (idx < (long)*count) {
+    const long c = list[idx];
+    const WS w = cell_ws<M>(a.ws, c);
+    const CM mat = cell_mat<M>(a.mat, c);
+    const double tq4 = lane_field<M, double>(w, offsetof(Lane<M>, tq) + 4 * sizeof(double));
+    int it = 0;
+    const int fail = gmres_cell<M>(w, mat, a.gmres_maxl, LANE_FIELD(M, w, gamma), tq4, LANE_FIELD(M, w, m),
+                                   LANE_FIELD(M, w, teval), LANE_FIELD(M, w, rE0), LANE_FIELD(M, w, fH), it);
+    mat[MS::IRF] = (double)fail;
+    mat[MS::IRS] = (double)it;
+    nli = (unsigned long long)it;
+  }
+  if (a.counters) wave_add(&a.counters[CNT_NLI], nli);
+}
+
 // advance every active cell's state machine by this trip's evaluation
 template <class M>
 __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_CONSUME_WAVES) k_consume(const ReactArgs a, const int* __restrict__ list,
@@ -2188,6 +2415,7 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(OCC) k_persist(const Rea
   st.lu_mode = a.lu_mode;
   st.ir_sweeps = a.ir_sweeps;
   st.use_pred = 0;
+  st.gmres = 0;   // the persistent kernel solves with the factorisation (GMRES runs in trips)
   st.rtol = a.rtol;
   st.mxstep = a.mxstep;
   st.tout = a.dt;
@@ -2768,6 +2996,11 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
 // so the grid strides
 template <class M>
 hipError_t launch_solve(const ReactArgs& a, long n, hipStream_t stream) {
+  if (a.gmres) {
+    hipLaunchKernelGGL(k_solve_gmres<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, a.nlist,
+                       &a.counts[6]);
+    return hipGetLastError();
+  }
   if (a.lu_lane) {
     hipLaunchKernelGGL(k_solve_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, a.nlist,
                        &a.counts[6]);
@@ -2860,7 +3093,7 @@ hipError_t enqueue_fixed_trip(const ReactArgs& a, int cur, hipStream_t s) {
   PELE_TRY(launch_lu<M>(a, a.slist[cur], &a.counts[2 + cur], n, s, false));
   // the cooperative solve strides over its list from a fixed grid; the one-lane solve
   // takes one cell per lane, so its grid covers every cell the list can hold
-  PELE_TRY(launch_solve<M>(a, a.lu_lane ? n : 16384, s));
+  PELE_TRY(launch_solve<M>(a, (a.lu_lane || a.gmres) ? n : 16384, s));
   if (a.compact)
     hipLaunchKernelGGL(k_consume<M>, dim3(nb), dim3(64), 0, s, a, act, act_n, a.list[nxt], &a.counts[nxt],
                        a.slist[nxt], &a.counts[2 + nxt]);

@@ -50,6 +50,9 @@ struct ReactArgs {
                                          // per wave, blocked; the solves stay cooperative)
   int lu_lane;                           // 1: one lane factors a cell (the form that runs when
                                          // the kernels execute serially on a CPU); 0: 32 lanes
+  int gmres;                             // 1: cvode.solve_type = GMRES (no matrix: the Newton
+                                         // systems by scaled GMRES with difference-quotient J v)
+  int gmres_maxl;                        // its Krylov dimension (1 .. N1 - 1)
   // per-cell workspace, tiled by 64 cells: element (slot s) of cell c at
   // ws[((c / 64) * nslot + s) * 64 + c % 64]
   double* ws;
@@ -75,7 +78,7 @@ struct ReactArgs {
 
 enum CounterIndex : int {
   CNT_CELLS = 0, CNT_NST, CNT_NFE, CNT_NJE, CNT_NSETUPS, CNT_NNI, CNT_NCFN, CNT_NETF, CNT_NFAIL,
-  CNT_MAXSTEPS, CNT_IR, CNT_LUFALLBACK, CNT_HY2T, CNT_PRED, CNT_PREDFB, NCOUNTERS = 16
+  CNT_MAXSTEPS, CNT_IR, CNT_LUFALLBACK, CNT_HY2T, CNT_PRED, CNT_PREDFB, CNT_NLI, NCOUNTERS = 16
 };
 
 // Mechanism ids (order of the dispatch table in reactor_bdf.hip).

@@ -41,7 +41,8 @@ class MiStats(C.Structure):
                 ("ms_kernel", C.c_double), ("ntrips", C.c_long), ("nhy2t_iters", C.c_long),
                 ("kernel_ms", C.c_double * 6),
                 ("kernel_launches", C.c_long * 6), ("kernel_cells", C.c_long * 6),
-                ("schedule_used", C.c_long), ("graph_trips_used", C.c_long), ("tune_ms", C.c_double * 4)]
+                ("schedule_used", C.c_long), ("graph_trips_used", C.c_long), ("tune_ms", C.c_double * 4),
+                ("nli", C.c_long)]
 
 
 # Every exported entry point of include/pelelm_chem.h.

@@ -10,7 +10,7 @@ thread_local Idx3 threadIdx, blockIdx, blockDim, gridDim;
 
 using namespace pelelm;
 
-static int g_lu_mode = LU_FP64, g_ir_sweeps = 3, g_pred = 0, g_fill = -1;
+static int g_lu_mode = LU_FP64, g_ir_sweeps = 3, g_pred = 0, g_fill = -1, g_gmres = 0, g_maxl = 5;
 static const int* g_order = nullptr;
 static const double *g_fy = nullptr, *g_fh = nullptr;
 static const int* g_mask = nullptr;
@@ -32,6 +32,13 @@ extern "C" void emu_set_pred(int on) { g_pred = on; }
 extern "C" void emu_set_lu(int mode, int ir_sweeps) {
   g_lu_mode = mode;
   g_ir_sweeps = ir_sweeps;
+}
+
+// 1: the following emu_react calls solve the Newton systems by GMRES (cvode.solve_type =
+// GMRES) with maxl Krylov vectors
+extern "C" void emu_set_gmres(int on, int maxl) {
+  g_gmres = on;
+  g_maxl = maxl;
 }
 
 extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, double* rE, double* FC, double dt,
@@ -74,6 +81,8 @@ extern "C" int emu_react(const char* mech, long ncell, double* rY, double* T, do
   a.lu_mode = g_lu_mode;
   a.ir_sweeps = g_ir_sweeps;
   a.ir_tol = 1e-4;
+  a.gmres = g_gmres;
+  a.gmres_maxl = g_maxl;
   a.counters = counters;
   a.slot_cell = g_order;
   if (g_fy) { a.rYsrc.p = const_cast<double*>(g_fy); a.rYsrc.ns = ncell; }

@@ -50,6 +50,8 @@ ReactArgs make_args(Host& h, int m, long ncell, long slots, double* rY, double* 
   a.counts = h.counts;
   a.lu_lane = 0;
   a.lu_mfma = g_lu_mfma;
+  a.gmres = 0;
+  a.gmres_maxl = 5;
   a.lu_mode = lu_mode;
   a.ir_sweeps = 3;
   a.ir_tol = 1e-4;

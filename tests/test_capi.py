@@ -55,7 +55,8 @@ def test_init_rejects_e_reactor():
 
 
 @pytest.mark.parametrize("key,value", [("ode.rtol", "-1"), ("ode.rtol", "abc"), ("cvode.max_order", "7"),
-                                       ("cvode.solve_type", "GMRES"), ("mi.mechanism", "gri30"),
+                                       ("cvode.solve_type", "precGMRES"), ("mi.gmres_maxl", "0"),
+                                       ("mi.gmres_maxl", "65"), ("mi.mechanism", "gri30"),
                                        ("mi.schedule", "fifo"), ("mi.graph_trips", "3"), ("mi.graph_trips", "-2"),
                                        ("mi.graph_trips", "66"), ("mi.lu", "fp16"), ("mi.lu_form", "warp"),
                                        ("no.such.key", "1")])
@@ -72,7 +73,8 @@ def test_set_accepts_reference_deck_keys():
                  ("cvode.max_order", "4"), ("peleLM.use_typ_vals_chem", "1"), ("mi.mechanism", "LiDryer"),
                  ("mi.schedule", "plain"), ("mi.schedule", "queue"), ("mi.schedule", "auto"),
                  ("mi.graph_trips", "8"), ("mi.graph_trips", "0"), ("mi.lu", "fp32_ir"), ("mi.lu", "fp64"),
-                 ("mi.lu_form", "lane"), ("mi.lu_form", "coop"), ("mi.ir_sweeps", "2")]:
+                 ("mi.lu_form", "lane"), ("mi.lu_form", "coop"), ("mi.lu_form", "mfma"), ("mi.ir_sweeps", "2"),
+                 ("cvode.solve_type", "GMRES"), ("mi.gmres_maxl", "8"), ("cvode.solve_type", "dense_direct")]:
         r.set(k, v)
     assert r.nspec == 9
     r.set_typ_vals_ode([1e-3] * 9 + [1500.0])

