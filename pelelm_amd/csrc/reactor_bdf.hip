@@ -1977,15 +1977,43 @@ __global__ void __launch_bounds__(64) k_solve_coop(const ReactArgs a, const int*
 // left in the workspace.  A residual that fell but did not converge is accepted on the
 // first Newton iteration only.  Returns 1 when the Newton iteration must fail.
 // Storage: the Krylov vectors in the workspace's Jacobian slots (unused without a
-// matrix), the Hessenberg matrix in the matrix region's factor block, the rotations in
-// its refinement vectors R and D, the small solution in the TMP slots.
+// matrix); the Hessenberg matrix, rotations and small solution in LDS up to 5 Krylov
+// vectors, beyond in the matrix region and TMP slots.
 // ---------------------------------------------------------------------------
 constexpr double GMRES_EPLIFAC = 0.05, GMRES_GS_FACTOR = 1000.0;
+// Krylov dimensions up to this keep the Hessenberg matrix, rotations and small solution
+// in LDS (46 words per lane at 5); larger ones keep them in the cell's matrix region
+constexpr int GMRES_LDS_MAXL = 5;
+constexpr int GMRES_LDS_WORDS = (GMRES_LDS_MAXL + 1) * GMRES_LDS_MAXL + 2 * GMRES_LDS_MAXL + GMRES_LDS_MAXL + 1;
 
-template <class M>
-__device__ __forceinline__ int gmres_cell(const WS& w, const CM& mat, const int maxl, const double gamma,
-                                          const double tq4, const int mnewt, const double t, const double rE0,
-                                          const double fH, int& nli) {
+// the small GMRES arrays of one lane: H (maxl + 1 rows of maxl), the rotations (c, s
+// pairs) and the small solution y.  In LDS, word k of lane l at sm[k * 64 + l] (the 64
+// lanes of an access hit 64 banks); in the matrix region, H in the factor block (row
+// stride N1), the rotations in R and D, y in the TMP slots of the workspace.
+struct GmresLds {
+  double* sm;
+  int lane;
+  __device__ __forceinline__ double& H(int i, int j) const { return sm[(i * GMRES_LDS_MAXL + j) * 64 + lane]; }
+  __device__ __forceinline__ double& G(int k) const {
+    return sm[((GMRES_LDS_MAXL + 1) * GMRES_LDS_MAXL + k) * 64 + lane];
+  }
+  __device__ __forceinline__ double& Y(int k) const {
+    return sm[((GMRES_LDS_MAXL + 1) * GMRES_LDS_MAXL + 2 * GMRES_LDS_MAXL + k) * 64 + lane];
+  }
+};
+template <int N1>
+struct GmresMat {
+  CM mat;
+  WS w;
+  __device__ __forceinline__ double& H(int i, int j) const { return mat[MatSlots<N1>::LU + i * N1 + j]; }
+  __device__ __forceinline__ double& G(int k) const { return mat[MatSlots<N1>::R + k]; }
+  __device__ __forceinline__ double& Y(int k) const { return w[Slots<N1>::TMP + k]; }
+};
+
+template <class M, class HS>
+__device__ __forceinline__ int gmres_cell(const WS& w, const CM& mat, const HS& hs, const int maxl,
+                                          const double gamma, const double tq4, const int mnewt, const double t,
+                                          const double rE0, const double fH, int& nli) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
   using MS = MatSlots<N1>;
@@ -2041,29 +2069,35 @@ __device__ __forceinline__ int gmres_cell(const WS& w, const CM& mat, const int 
     for (int i = 0; i < N1; ++i) work[i] = sig * vt[i] + w[S::Y + i];
     rhs<M>(work, t, w, rE0, fH, fw, Tw);
     const double siginv = 1.0 / sig;
+    // the new vector stays in registers through the orthogonalisation; each basis
+    // vector is read once per pass
+    double z[N1];
     double vn2 = 0.0;
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       const double jv = siginv * fw[i] - siginv * w[S::F + i];
-      const double z = w[S::EWT + i] * (vt[i] - gamma * jv);
-      w[vk + i] = z;
-      vn2 += z * z;
+      z[i] = w[S::EWT + i] * (vt[i] - gamma * jv);
+      vn2 += z[i] * z[i];
     }
     // modified Gram-Schmidt against V[0..l]
     const double vk_norm = sqrt(vn2);
 #pragma unroll 1
     for (int j = 0; j <= l; ++j) {
       const int vj = S::JS + j * N1;
+      double v[N1];
       double h = 0.0;
 #pragma unroll
-      for (int i = 0; i < N1; ++i) h += w[vj + i] * w[vk + i];
-      mat[MS::LU + j * N1 + l] = h;
+      for (int i = 0; i < N1; ++i) {
+        v[i] = w[vj + i];
+        h += v[i] * z[i];
+      }
+      hs.H(j, l) = h;
 #pragma unroll
-      for (int i = 0; i < N1; ++i) w[vk + i] = w[vk + i] - h * w[vj + i];
+      for (int i = 0; i < N1; ++i) z[i] = z[i] - h * v[i];
     }
     double nrm2 = 0.0;
 #pragma unroll
-    for (int i = 0; i < N1; ++i) nrm2 += w[vk + i] * w[vk + i];
+    for (int i = 0; i < N1; ++i) nrm2 += z[i] * z[i];
     double nrm = sqrt(nrm2);
     const double big = GMRES_GS_FACTOR * vk_norm;
     if (big + nrm == big) {
@@ -2071,26 +2105,30 @@ __device__ __forceinline__ int gmres_cell(const WS& w, const CM& mat, const int 
 #pragma unroll 1
       for (int j = 0; j <= l; ++j) {
         const int vj = S::JS + j * N1;
+        double v[N1];
         double tdot = 0.0;
 #pragma unroll
-        for (int i = 0; i < N1; ++i) tdot += w[vj + i] * w[vk + i];
+        for (int i = 0; i < N1; ++i) {
+          v[i] = w[vj + i];
+          tdot += v[i] * z[i];
+        }
         newp += tdot * tdot;
-        mat[MS::LU + j * N1 + l] += tdot;
+        hs.H(j, l) += tdot;
 #pragma unroll
-        for (int i = 0; i < N1; ++i) w[vk + i] = w[vk + i] - tdot * w[vj + i];
+        for (int i = 0; i < N1; ++i) z[i] = z[i] - tdot * v[i];
       }
       if (newp != 0.0) nrm = sqrt(nrm * nrm - newp);
     }
-    mat[MS::LU + (l + 1) * N1 + l] = nrm;
+    hs.H(l + 1, l) = nrm;
     // the earlier rotations on column l, then the rotation that zeroes H[l+1][l]
 #pragma unroll 1
     for (int i = 0; i < l; ++i) {
-      const double c = mat[MS::R + 2 * i], sn = mat[MS::R + 2 * i + 1];
-      const double t1 = mat[MS::LU + i * N1 + l], t2 = mat[MS::LU + (i + 1) * N1 + l];
-      mat[MS::LU + i * N1 + l] = c * t1 - sn * t2;
-      mat[MS::LU + (i + 1) * N1 + l] = sn * t1 + c * t2;
+      const double c = hs.G(2 * i), sn = hs.G(2 * i + 1);
+      const double t1 = hs.H(i, l), t2 = hs.H(i + 1, l);
+      hs.H(i, l) = c * t1 - sn * t2;
+      hs.H(i + 1, l) = sn * t1 + c * t2;
     }
-    const double t1 = mat[MS::LU + l * N1 + l], t2 = mat[MS::LU + (l + 1) * N1 + l];
+    const double t1 = hs.H(l, l), t2 = nrm;
     double c, sn;
     if (t2 == 0.0) {
       c = 1.0;
@@ -2104,10 +2142,10 @@ __device__ __forceinline__ int gmres_cell(const WS& w, const CM& mat, const int 
       c = 1.0 / sqrt(1.0 + t3 * t3);
       sn = -c * t3;
     }
-    mat[MS::R + 2 * l] = c;
-    mat[MS::R + 2 * l + 1] = sn;
+    hs.G(2 * l) = c;
+    hs.G(2 * l + 1) = sn;
     const double hll = c * t1 - sn * t2;
-    mat[MS::LU + l * N1 + l] = hll;
+    hs.H(l, l) = hll;
     if (hll == 0.0) return 1;   // breakdown of the Givens-reduced Hessenberg matrix
     rotp *= sn;
     rho = fabs(rotp * beta);
@@ -2117,35 +2155,35 @@ __device__ __forceinline__ int gmres_cell(const WS& w, const CM& mat, const int 
     }
     const double rinv = 1.0 / nrm;
 #pragma unroll
-    for (int i = 0; i < N1; ++i) w[vk + i] *= rinv;
+    for (int i = 0; i < N1; ++i) w[vk + i] = z[i] * rinv;
   }
   if (!converged && !(rho < beta)) return 1;
   if (!converged && mnewt > 0) return 1;
-  // y = R^-1 Q^T (beta e1) in the TMP slots, x = S^-1 V y
-  w[S::TMP] = beta;
+  // y = R^-1 Q^T (beta e1), x = S^-1 V y
+  hs.Y(0) = beta;
 #pragma unroll 1
-  for (int i = 1; i <= krydim; ++i) w[S::TMP + i] = 0.0;
+  for (int i = 1; i <= krydim; ++i) hs.Y(i) = 0.0;
 #pragma unroll 1
   for (int k = 0; k < krydim; ++k) {
-    const double c = mat[MS::R + 2 * k], sn = mat[MS::R + 2 * k + 1];
-    const double a1 = w[S::TMP + k], a2 = w[S::TMP + k + 1];
-    w[S::TMP + k] = c * a1 - sn * a2;
-    w[S::TMP + k + 1] = sn * a1 + c * a2;
+    const double c = hs.G(2 * k), sn = hs.G(2 * k + 1);
+    const double a1 = hs.Y(k), a2 = hs.Y(k + 1);
+    hs.Y(k) = c * a1 - sn * a2;
+    hs.Y(k + 1) = sn * a1 + c * a2;
   }
 #pragma unroll 1
   for (int k = krydim - 1; k >= 0; --k) {
-    const double hkk = mat[MS::LU + k * N1 + k];
+    const double hkk = hs.H(k, k);
     if (hkk == 0.0) return 1;
-    const double yk = w[S::TMP + k] / hkk;
-    w[S::TMP + k] = yk;
+    const double yk = hs.Y(k) / hkk;
+    hs.Y(k) = yk;
 #pragma unroll 1
-    for (int i = 0; i < k; ++i) w[S::TMP + i] -= yk * mat[MS::LU + i * N1 + k];
+    for (int i = 0; i < k; ++i) hs.Y(i) -= yk * hs.H(i, k);
   }
 #pragma unroll
   for (int i = 0; i < N1; ++i) {
-    double x = w[S::TMP] * w[S::JS + i];
+    double x = hs.Y(0) * w[S::JS + i];
 #pragma unroll 1
-    for (int k = 1; k < krydim; ++k) x += w[S::TMP + k] * w[S::JS + k * N1 + i];
+    for (int k = 1; k < krydim; ++k) x += hs.Y(k) * w[S::JS + k * N1 + i];
     mat[MS::X + i] = x / w[S::EWT + i];
   }
   return 0;
@@ -2158,6 +2196,7 @@ template <class M>
 __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_EVAL_WAVES(M::N1)) k_solve_gmres(
     const ReactArgs a, const int* __restrict__ list, const unsigned int* __restrict__ count) {
   using MS = MatSlots<M::N1>;
+  __shared__ double sg[GMRES_LDS_WORDS * 64];
   const long idx = (long)blockIdx.x * 64 + threadIdx.x;
   unsigned long long nli = 0;
   if (idx < (long)*count) {
@@ -2165,9 +2204,15 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_EVAL_WAVES(M::N1)) 
     const WS w = cell_ws<M>(a.ws, c);
     const CM mat = cell_mat<M>(a.mat, c);
     const double tq4 = lane_field<M, double>(w, offsetof(Lane<M>, tq) + 4 * sizeof(double));
-    int it = 0;
-    const int fail = gmres_cell<M>(w, mat, a.gmres_maxl, LANE_FIELD(M, w, gamma), tq4, LANE_FIELD(M, w, m),
-                                   LANE_FIELD(M, w, teval), LANE_FIELD(M, w, rE0), LANE_FIELD(M, w, fH), it);
+    const double gamma = LANE_FIELD(M, w, gamma), teval = LANE_FIELD(M, w, teval);
+    const double rE0 = LANE_FIELD(M, w, rE0), fH = LANE_FIELD(M, w, fH);
+    const int mnewt = LANE_FIELD(M, w, m);
+    int it = 0, fail;
+    if (a.gmres_maxl <= GMRES_LDS_MAXL)
+      fail = gmres_cell<M>(w, mat, GmresLds{sg, (int)threadIdx.x}, a.gmres_maxl, gamma, tq4, mnewt, teval, rE0, fH,
+                           it);
+    else
+      fail = gmres_cell<M>(w, mat, GmresMat<M::N1>{mat, w}, a.gmres_maxl, gamma, tq4, mnewt, teval, rE0, fH, it);
     mat[MS::IRF] = (double)fail;
     mat[MS::IRS] = (double)it;
     nli = (unsigned long long)it;
