@@ -6,7 +6,7 @@
 
 #include "chem_oracle.h"
 
-#define ORC_NMAX 64            // largest ODE size (species + 1) the oracle handles
+#define ORC_NMAX 96            // largest ODE size (species + 1) the oracle handles
 #define ORC_HY2T_MAXIT 50
 #define ORC_HY2T_RTOL 1e-10    // Newton on T stops once |dT| <= 1e-10 T
 

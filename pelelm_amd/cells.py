@@ -75,6 +75,8 @@ WORKLOADS = {
     "drm19_64": Workload("drm19", "CH4", 2.0, (0.6, 1.2), 0, 102, 64 ** 3, 1e-6),
     "drm19_128_mix": Workload("drm19", "CH4", 2.0, (0.6, 1.2), 1, 104, 128 ** 3, 1e-5),
     "dodecane53_64_mix": Workload("dodecane53", "NC12H26", 18.5, (0.6, 1.2), 1, 105, 64 ** 3, 1e-5),
+    "dodecane56_64_mix": Workload("dodecane56", "NC12H26", 18.5, (0.6, 1.2), 1, 106, 64 ** 3, 1e-5),
+    "heptane88_64_mix": Workload("heptane88", "NC7H16", 11.0, (0.6, 1.2), 1, 107, 64 ** 3, 1e-5),
 }
 
 

@@ -482,7 +482,7 @@ int autotune(Reactor* r, const ReactArgs& a0, hipStream_t stream, long max_trips
   } fr{{&rY, &T, &rE, &FC, &cY, &cT, &cE, &fY, &fE}};
   for (double** p : {&rY, &cY, &fY}) HIPCHK(r, hipMalloc(p, sizeof(double) * N * n));
   for (double** p : {&T, &rE, &FC, &cT, &cE, &fE}) HIPCHK(r, hipMalloc(p, sizeof(double) * n));
-  std::vector<TuneConfig> cand = tune_candidates(!r->gmres);
+  std::vector<TuneConfig> cand = tune_candidates(!r->gmres && mech_nspec(r->mech) + 1 <= COOP_MAX_N1);
   const long strides[3] = {16, 4, 1};
   const size_t keep[3] = {3, 2, 1};
   std::vector<float> last_ms;
@@ -556,6 +556,17 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   a.ir_tol = r->ir_tol;
   a.lu_lane = r->lu_lane;
   a.lu_mfma = r->lu_mfma;
+  const bool wide = N1 > COOP_MAX_N1;
+  if (wide && !r->rk64 && !r->null_reactor) {
+    // one matrix row per lane of a wave does not reach: one wave per cell with two rows
+    // per lane, FP64 factors (or one lane per cell with mi.lu_form = lane)
+    a.lu_mfma = 0;
+    a.lu_mode = LU_FP64;
+    if (fixed || r->schedule == SCHED_PERSIST)
+      return fail(r, MI_EINVAL, "mechanisms above 63 species run on the trip schedules (plain, queue, auto), not "
+                                "the persistent kernel");
+    if (r->predictor) return fail(r, MI_EINVAL, "the learned predictor takes at most 61 species");
+  }
   a.gmres = (r->gmres && !r->rk64 && !r->null_reactor) ? 1 : 0;
   a.gmres_maxl = r->gmres_maxl;
   if (a.gmres) {

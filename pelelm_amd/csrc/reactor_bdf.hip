@@ -52,9 +52,39 @@
 #define PELE_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
 
+// Build modes.  By default this file is one translation unit holding every mechanism and
+// the shared kernels.  The library builds it once per mechanism (PELE_TU_MECH set, with
+// PELE_TU_MECH_INC / _TYPE / _FN naming the generated code, its type and its table
+// function: the unit holds that mechanism's kernels and its entry table) and once with
+// PELE_TU_COMMON (the shared kernels and the dispatch over the tables), so the
+// mechanisms compile in parallel.
+// A single unit can leave out the two largest mechanisms (PELE_BASE_MECHS_ONLY: their
+// tables are then absent and mi.mechanism rejects them; PELE_WITH_HEPTANE88 keeps the
+// 88-species one), which keeps test builds short.
+#if defined(PELE_BASE_MECHS_ONLY)
+#define PELE_HAVE_DODECANE56 0
+#if defined(PELE_WITH_HEPTANE88)
+#define PELE_HAVE_HEPTANE88 1
+#else
+#define PELE_HAVE_HEPTANE88 0
+#endif
+#else
+#define PELE_HAVE_DODECANE56 1
+#define PELE_HAVE_HEPTANE88 1
+#endif
+#if defined(PELE_TU_MECH)
+#include PELE_TU_MECH_INC
+#elif !defined(PELE_TU_COMMON)
 #include "mech/LiDryer.inc"
 #include "mech/drm19.inc"
 #include "mech/dodecane53.inc"
+#if PELE_HAVE_DODECANE56
+#include "mech/dodecane56.inc"
+#endif
+#if PELE_HAVE_HEPTANE88
+#include "mech/heptane88.inc"
+#endif
+#endif
 
 namespace pelelm {
 namespace {
@@ -2988,6 +3018,158 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_EVAL_WAVES(M::N1)) 
 // Blocks for the cooperative kernels (CPW cells per block), a multiple of 8: with
 // `exact` (the count n is known) enough for every cell, else a fixed 8192 blocks that
 // stride over whatever the list holds.
+// ---------------------------------------------------------------------------
+// Mechanisms above 64 unknowns (one matrix row per lane of a wave does not reach): one
+// cell per wave, lane l owning rows l and l + 64.
+//
+// LU: M = I - gamma J staged in LDS (N1 x LD words, LD odd; 63.4 KB at 89 unknowns), the
+// column loop at run time, each step's pivot the largest |m_rk| over the rows not yet
+// pivoted (ties to the lowest row, a NaN as infinite: one 64-bit max over the wave and
+// a ballot, rows below 64 first), multipliers m_rk (1 / pivot) and the row updates from
+// the pivot row, read from LDS by every lane at once.  Rows stay in place; factor row s is
+// the row chosen at step s (PERM[s]), as the cooperative solves read them.  FP64 only.
+// ---------------------------------------------------------------------------
+template <int N1>
+struct Wide {
+  static constexpr int LD = N1 | 1;
+  static_assert(N1 <= 128, "two rows per lane");
+  static_assert(N1 * LD * 8 + N1 * 4 <= 65536, "the matrix must fit one workgroup's LDS");
+};
+
+template <class M>
+__device__ __forceinline__ void lu_wide_one(const ReactArgs& a, const int* __restrict__ list, const long n,
+                                            const long idx) {
+  constexpr int N1 = M::N1, LD = Wide<N1>::LD;
+  using S = Slots<N1>;
+  using MS = MatSlots<N1>;
+  __shared__ double sm[N1 * LD];
+  __shared__ int sstep[N1];
+  if (idx >= n) return;   // uniform over the block (one cell per block)
+  const long c = list[idx];
+  const WS w = cell_ws<M>(a.ws, c);
+  if (LANE_FIELD(M, w, setup_pending) == 0) return;
+  const double gamma = LANE_FIELD(M, w, gamma);
+  const CM mat = cell_mat<M>(a.mat, c);
+  const int l = threadIdx.x;
+#pragma unroll 1
+  for (int e = l; e < N1 * N1; e += 64) {
+    const int i = e / N1, j = e - (e / N1) * N1;
+    sm[i * LD + j] = (i == j ? 1.0 : 0.0) - gamma * mat[MS::JS + e];
+  }
+  const int r0 = l, r1 = l + 64;
+  bool done0 = r0 >= N1, done1 = r1 >= N1;
+  int sing = 0;
+  __syncthreads();
+#pragma unroll 1
+  for (int k = 0; k < N1; ++k) {
+    auto key_of = [&](int r, bool done) -> unsigned long long {
+      if (done) return 0ull;
+      double av = fabs(sm[r * LD + k]);
+      if (av != av) av = HUGE_VAL;
+      return (unsigned long long)__double_as_longlong(av) + 1ull;
+    };
+    const unsigned long long k0 = key_of(r0, done0), k1 = key_of(r1, done1);
+    unsigned long long kmax = k0 > k1 ? k0 : k1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(kmax, off, 64);
+      kmax = o > kmax ? o : kmax;
+    }
+    const unsigned long long b0 = __ballot(k0 == kmax);
+    const unsigned long long b1 = __ballot(k1 == kmax);
+    const int prow = b0 ? __ffsll((long long)b0) - 1 : 64 + __ffsll((long long)b1) - 1;
+    const bool nonzero = kmax > 1ull;
+    if (!nonzero && sing == 0) sing = k + 1;
+    if (nonzero) {
+      const double inv = 1.0 / sm[prow * LD + k];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = h ? r1 : r0;
+        const bool d = h ? done1 : done0;
+        if (d || r == prow) continue;
+        const double lik = sm[r * LD + k] * inv;
+        sm[r * LD + k] = lik;
+        if (lik != 0.0) {
+#pragma unroll 1
+          for (int j = k + 1; j < N1; ++j) sm[r * LD + j] -= lik * sm[prow * LD + j];
+        }
+      }
+    }
+    if (r0 == prow) done0 = true;
+    if (r1 == prow) done1 = true;
+    if (l == (prow & 63)) sstep[k] = prow;
+    __syncthreads();
+  }
+#pragma unroll 1
+  for (int e = l; e < N1 * N1; e += 64) {
+    const int st = e / N1, j = e - (e / N1) * N1;
+    mat[MS::LU + e] = sm[sstep[st] * LD + j];
+  }
+#pragma unroll 1
+  for (int st = l; st < N1; st += 64) mat.perm(MS::PERM, st) = sstep[st];
+  if (l == 0) w[S::SING] = (double)sing;
+  __syncthreads();   // the LDS is reused by the next cell of this block
+}
+
+template <class M, bool STRIDE>
+__global__ void __launch_bounds__(64) k_lu_wide(const ReactArgs a, const int* __restrict__ list,
+                                                const unsigned int* __restrict__ count) {
+  const long n = (long)*count;
+  if (STRIDE) {
+    for (long blk = xcd_block(); blk < n; blk += gridDim.x) lu_wide_one<M>(a, list, n, blk);
+  } else {
+    lu_wide_one<M>(a, list, n, xcd_block());
+  }
+}
+
+// The Newton solves for these cells: x = M^-1 B with the factors above, lane l holding
+// components l and l + 64; forward and back substitution column by column, each finished
+// component broadcast with v_readlane (factor rows read from the matrix region, a lane's
+// row walked in order through the cache).
+template <class M>
+__device__ __forceinline__ void solve_wide_one(const ReactArgs& a, const int* __restrict__ list, const long n,
+                                               const long idx) {
+  constexpr int N1 = M::N1;
+  using MS = MatSlots<N1>;
+  if (idx >= n) return;   // uniform over the block
+  const long c = list[idx];
+  const WS w = cell_ws<M>(a.ws, c);
+  if (!solve_due<M>(w)) return;
+  const CM mat = cell_mat<M>(a.mat, c);
+  const int l = threadIdx.x;
+  const int r0 = l, r1 = l + 64;
+  const bool has0 = r0 < N1, has1 = r1 < N1;
+  double y0 = has0 ? mat[MS::B + mat.perm(MS::PERM, r0)] : 0.0;
+  double y1 = has1 ? mat[MS::B + mat.perm(MS::PERM, r1)] : 0.0;
+#pragma unroll 1
+  for (int k = 0; k < N1 - 1; ++k) {
+    const double yk = lane_read(k < 64 ? y0 : y1, k & 63);
+    if (has0 && r0 > k) y0 -= mat[MS::LU + r0 * N1 + k] * yk;
+    if (has1 && r1 > k) y1 -= mat[MS::LU + r1 * N1 + k] * yk;
+  }
+#pragma unroll 1
+  for (int k = N1 - 1; k >= 0; --k) {
+    // the owner divides by the pivot; the quotient goes to every lane
+    if (k < 64) {
+      if (l == k) y0 = y0 / mat[MS::LU + k * N1 + k];
+    } else {
+      if (l == k - 64) y1 = y1 / mat[MS::LU + k * N1 + k];
+    }
+    const double xk = lane_read(k < 64 ? y0 : y1, k & 63);
+    if (has0 && r0 < k) y0 -= mat[MS::LU + r0 * N1 + k] * xk;
+    if (has1 && r1 < k) y1 -= mat[MS::LU + r1 * N1 + k] * xk;
+  }
+  if (has0) mat[MS::X + r0] = y0;
+  if (has1) mat[MS::X + r1] = y1;
+}
+
+template <class M>
+__global__ void __launch_bounds__(64) k_solve_wide(const ReactArgs a, const int* __restrict__ list,
+                                                   const unsigned int* __restrict__ count) {
+  const long n = (long)*count;
+  for (long blk = xcd_block(); blk < n; blk += gridDim.x) solve_wide_one<M>(a, list, n, blk);
+}
+
 template <int N1>
 inline unsigned coop_grid(long n, bool exact) {
   constexpr int CPW = Coop<N1>::CPW;
@@ -3004,6 +3186,13 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
     hipLaunchKernelGGL(k_lu_lane<M>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, a, list, count);
     return hipGetLastError();
   }
+  if constexpr (M::N1 > COOP_MAX_N1) {   // one cell per block; a multiple of 8 blocks (XCD mapping)
+    const dim3 g(exact ? (unsigned)(n > 0 ? ((n + 7) / 8) * 8 : 8) : 8192u);
+    if (exact)
+      hipLaunchKernelGGL((k_lu_wide<M, false>), g, dim3(64), 0, stream, a, list, count);
+    else
+      hipLaunchKernelGGL((k_lu_wide<M, true>), g, dim3(64), 0, stream, a, list, count);
+  } else {
   if (a.lu_mfma) {   // one cell per block; a multiple of 8 blocks (XCD mapping)
     const dim3 g(exact ? (unsigned)(n > 0 ? ((n + 7) / 8) * 8 : 8) : 8192u);
     if (a.lu_mode == LU_FP32_IR) {
@@ -3034,6 +3223,7 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
     hipLaunchKernelGGL((k_lu_coop<M, double, false>), grid, dim3(64), 0, stream, a, list, count);
   else
     hipLaunchKernelGGL((k_lu_coop<M, double, true>), grid, dim3(64), 0, stream, a, list, count);
+  }
   return hipGetLastError();
 }
 
@@ -3051,13 +3241,19 @@ hipError_t launch_solve(const ReactArgs& a, long n, hipStream_t stream) {
                        &a.counts[6]);
     return hipGetLastError();
   }
-  const dim3 grid(coop_grid<M::N1>(n, true));
-  if (a.lu_mode == LU_FP32_IR) {
-    hipLaunchKernelGGL((k_solve_coop<M, float, true>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+  if constexpr (M::N1 > COOP_MAX_N1) {   // one cell per block over the solve list
+    const long nb = n < 8192 ? n : 8192;
+    hipLaunchKernelGGL(k_solve_wide<M>, dim3((unsigned)(nb > 0 ? ((nb + 7) / 8) * 8 : 8)), dim3(64), 0, stream, a,
+                       a.nlist, &a.counts[6]);
+  } else {
+    const dim3 grid(coop_grid<M::N1>(n, true));
+    if (a.lu_mode == LU_FP32_IR) {
+      hipLaunchKernelGGL((k_solve_coop<M, float, true>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_solve_coop<M, double, true>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
   }
-  hipLaunchKernelGGL((k_solve_coop<M, double, true>), grid, dim3(64), 0, stream, a, a.nlist, &a.counts[6]);
   return hipGetLastError();
 }
 
@@ -3107,12 +3303,16 @@ struct Timer {
 
 template <class M>
 hipError_t launch_predict(const ReactArgs& a, const MlpDev& w, unsigned np, hipStream_t stream) {
-  constexpr int KIN = (M::NS + 3 <= 32) ? 32 : 64;
-  constexpr int NOUT = (M::N1 <= 32) ? 32 : 64;
-  if (w.n_in != M::NS + 3 || w.n_out != M::N1 || w.kp[0] != KIN || w.np[3] != NOUT) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_predict<M, KIN, NOUT>), dim3((np + MLP_ROWS - 1) / MLP_ROWS), dim3(MLP_THREADS), 0, stream, a,
-                     w, a.plist, &a.counts[7]);
-  return hipGetLastError();
+  if constexpr (M::NS + 3 > 64 || M::N1 > 64) {   // the network's layers hold at most 64 inputs and outputs
+    return hipErrorNotSupported;
+  } else {
+    constexpr int KIN = (M::NS + 3 <= 32) ? 32 : 64;
+    constexpr int NOUT = (M::N1 <= 32) ? 32 : 64;
+    if (w.n_in != M::NS + 3 || w.n_out != M::N1 || w.kp[0] != KIN || w.np[3] != NOUT) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_predict<M, KIN, NOUT>), dim3((np + MLP_ROWS - 1) / MLP_ROWS), dim3(MLP_THREADS), 0, stream,
+                       a, w, a.plist, &a.counts[7]);
+    return hipGetLastError();
+  }
 }
 
 // One trip with grids sized for every cell (kernels skip beyond their list counts), as
@@ -3311,9 +3511,13 @@ hipError_t run_react_t(ReactArgs a, hipStream_t stream, long max_trips, long* tr
 template <class M, int OCC>
 hipError_t launch_persist_occ(const ReactArgs& a, const int* queue, const PersistPlan& plan, long max_trips,
                               hipStream_t s) {
-  hipLaunchKernelGGL((k_persist<M, OCC>), dim3((unsigned)plan.waves), dim3(64), 0, s, a, queue, &a.counts[0],
-                     &a.counts[1], &a.counts[2], plan.chunk, max_trips);
-  return hipGetLastError();
+  if constexpr (M::N1 > COOP_MAX_N1) {   // the persistent kernel's linear algebra is cooperative
+    return hipErrorNotSupported;
+  } else {
+    hipLaunchKernelGGL((k_persist<M, OCC>), dim3((unsigned)plan.waves), dim3(64), 0, s, a, queue, &a.counts[0],
+                       &a.counts[1], &a.counts[2], plan.chunk, max_trips);
+    return hipGetLastError();
+  }
 }
 
 // counts[0]: queue length, [1]: queue head, [2]: most iterations of a wave
@@ -3337,14 +3541,19 @@ hipError_t run_react_persist_t(const ReactArgs& a, const PersistPlan& plan, long
 
 template <class M>
 int persist_capacity_t(int occupancy) {
-  int dev = 0, ncu = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  const hipError_t e = occupancy == 2
-                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist<M, 2>, 64, 0)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist<M, 1>, 64, 0);
-  if (e != hipSuccess) return 0;
-  return ncu * per_cu;
+  if constexpr (M::N1 > COOP_MAX_N1) {   // no persistent kernel for this mechanism
+    (void)occupancy;
+    return 0;
+  } else {
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const hipError_t e = occupancy == 2
+                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist<M, 2>, 64, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist<M, 1>, 64, 0);
+    if (e != hipSuccess) return 0;
+    return ncu * per_cu;
+  }
 }
 
 // PeleLM::RhoH_to_Temp per cell (getTfromHY, Source/PeleLM_K.H:511-530): Y = rhoY / rho
@@ -3395,21 +3604,66 @@ hipError_t launch_rk64_t(const ReactArgs& a, const RK64Opts& o, hipStream_t stre
 
 }  // namespace
 
-int mech_nspec(int mech) {
+// one mechanism's entry table: its sizes and its kernels' launchers
+#define PELE_DEFINE_MECH_TABLE(T, FN)                                                                            \
+  const MechTable* FN() {                                                                                      \
+    static const MechTable t = {                                                                               \
+        T::NS, T::NAME, Slots<T::N1>::TOTAL, Slots<T::N1>::ITOTAL, MatSlots<T::N1>::STRIDE,                    \
+        [](const ReactArgs& a, hipStream_t s, long mt, long* to, KernelProf* p, const MlpDev* pr, TripPlan pl) { \
+          return run_react_t<T>(a, s, mt, to, p, pr, pl);                                                      \
+        },                                                                                                     \
+        [](const ReactArgs& a, const PersistPlan& pl, long mt, hipStream_t s) {                                \
+          return run_react_persist_t<T>(a, pl, mt, s);                                                         \
+        },                                                                                                     \
+        [](int occ) { return persist_capacity_t<T>(occ); },                                                    \
+        [](const RatesArgs& ra, hipStream_t s) { return launch_rates_t<T>(ra, s); },                           \
+        [](const ReactArgs& a, int g, int mn, int mx, hipStream_t s) {                                         \
+          return launch_rk64_t<T>(a, RK64Opts{g, mn, mx}, s);                                                  \
+        },                                                                                                     \
+        [](const ReactArgs& a, int mode, unsigned int* keys, int* vals, hipStream_t s) {                       \
+          return launch_cost_keys_t<T>(a, mode, keys, vals, s);                                                \
+        },                                                                                                     \
+        [](const ArrView& rho, const ArrView& rhoY, const ArrView& rhoH, const ArrView& Tv, int nx, int ny,    \
+           long ncell, hipStream_t s) { return launch_rhoh_to_temp_t<T>(rho, rhoY, rhoH, Tv, nx, ny, ncell, s); }}; \
+    return &t;                                                                                                 \
+  }
+
+#if defined(PELE_TU_MECH)
+PELE_DEFINE_MECH_TABLE(PELE_TU_MECH_TYPE, PELE_TU_MECH_FN)
+#else
+#if !defined(PELE_TU_COMMON)
+PELE_DEFINE_MECH_TABLE(mech_LiDryer, mech_table_LiDryer)
+PELE_DEFINE_MECH_TABLE(mech_drm19, mech_table_drm19)
+PELE_DEFINE_MECH_TABLE(mech_dodecane53, mech_table_dodecane53)
+#if PELE_HAVE_DODECANE56
+PELE_DEFINE_MECH_TABLE(mech_dodecane56, mech_table_dodecane56)
+#endif
+#if PELE_HAVE_HEPTANE88
+PELE_DEFINE_MECH_TABLE(mech_heptane88, mech_table_heptane88)
+#endif
+#endif
+
+const MechTable* mech_table(int mech) {
   switch (mech) {
-    case MECH_LIDRYER: return mech_LiDryer::NS;
-    case MECH_DRM19: return mech_drm19::NS;
-    case MECH_DODECANE53: return mech_dodecane53::NS;
-    default: return -1;
+    case MECH_LIDRYER: return mech_table_LiDryer();
+    case MECH_DRM19: return mech_table_drm19();
+    case MECH_DODECANE53: return mech_table_dodecane53();
+#if PELE_HAVE_DODECANE56 || defined(PELE_TU_COMMON)
+    case MECH_DODECANE56: return mech_table_dodecane56();
+#endif
+#if PELE_HAVE_HEPTANE88 || defined(PELE_TU_COMMON)
+    case MECH_HEPTANE88: return mech_table_heptane88();
+#endif
+    default: return nullptr;
   }
 }
+int mech_nspec(int mech) {
+  const MechTable* t = mech_table(mech);
+  return t ? t->nspec : -1;
+}
 const char* mech_name(int mech) {
-  switch (mech) {
-    case MECH_LIDRYER: return mech_LiDryer::NAME;
-    case MECH_DRM19: return mech_drm19::NAME;
-    case MECH_DODECANE53: return mech_dodecane53::NAME;
-    default: return "";
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->name : "";
 }
 int mech_lookup(const char* name) {
   for (int m = 0; m < NMECH; ++m) {
@@ -3421,98 +3675,50 @@ int mech_lookup(const char* name) {
   return -1;
 }
 long ws_slots(int mech) {
-  switch (mech) {
-    case MECH_LIDRYER: return Slots<mech_LiDryer::N1>::TOTAL;
-    case MECH_DRM19: return Slots<mech_drm19::N1>::TOTAL;
-    case MECH_DODECANE53: return Slots<mech_dodecane53::N1>::TOTAL;
-    default: return 0;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->ws_slots : 0;
 }
 long iws_slots(int mech) {
-  switch (mech) {
-    case MECH_LIDRYER: return Slots<mech_LiDryer::N1>::ITOTAL;
-    case MECH_DRM19: return Slots<mech_drm19::N1>::ITOTAL;
-    case MECH_DODECANE53: return Slots<mech_dodecane53::N1>::ITOTAL;
-    default: return 0;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->iws_slots : 0;
 }
-
 long mat_stride(int mech) {
-  switch (mech) {
-    case MECH_LIDRYER: return MatSlots<mech_LiDryer::N1>::STRIDE;
-    case MECH_DRM19: return MatSlots<mech_drm19::N1>::STRIDE;
-    case MECH_DODECANE53: return MatSlots<mech_dodecane53::N1>::STRIDE;
-    default: return 0;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->mat_stride : 0;
 }
 
 hipError_t run_react(int mech, const ReactArgs& a, hipStream_t stream, long max_trips, long* trips_out,
                      KernelProf* prof, const MlpDev* pred, TripPlan plan) {
-  switch (mech) {
-    case MECH_LIDRYER: return run_react_t<mech_LiDryer>(a, stream, max_trips, trips_out, prof, pred, plan);
-    case MECH_DRM19: return run_react_t<mech_drm19>(a, stream, max_trips, trips_out, prof, pred, plan);
-    case MECH_DODECANE53: return run_react_t<mech_dodecane53>(a, stream, max_trips, trips_out, prof, pred, plan);
-    default: return hipErrorInvalidValue;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->run_react(a, stream, max_trips, trips_out, prof, pred, plan) : hipErrorInvalidValue;
 }
-
 hipError_t run_react_persist(int mech, const ReactArgs& a, const PersistPlan& plan, long max_trips,
                              hipStream_t stream) {
-  switch (mech) {
-    case MECH_LIDRYER: return run_react_persist_t<mech_LiDryer>(a, plan, max_trips, stream);
-    case MECH_DRM19: return run_react_persist_t<mech_drm19>(a, plan, max_trips, stream);
-    case MECH_DODECANE53: return run_react_persist_t<mech_dodecane53>(a, plan, max_trips, stream);
-    default: return hipErrorInvalidValue;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->run_react_persist(a, plan, max_trips, stream) : hipErrorInvalidValue;
 }
-
 int persist_capacity(int mech, int occupancy) {
-  switch (mech) {
-    case MECH_LIDRYER: return persist_capacity_t<mech_LiDryer>(occupancy);
-    case MECH_DRM19: return persist_capacity_t<mech_drm19>(occupancy);
-    case MECH_DODECANE53: return persist_capacity_t<mech_dodecane53>(occupancy);
-    default: return 0;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->persist_capacity(occupancy) : 0;
 }
-
 hipError_t launch_rates(int mech, const RatesArgs& ra, hipStream_t stream) {
-  switch (mech) {
-    case MECH_LIDRYER: return launch_rates_t<mech_LiDryer>(ra, stream);
-    case MECH_DRM19: return launch_rates_t<mech_drm19>(ra, stream);
-    case MECH_DODECANE53: return launch_rates_t<mech_dodecane53>(ra, stream);
-    default: return hipErrorInvalidValue;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->launch_rates(ra, stream) : hipErrorInvalidValue;
 }
-
 hipError_t launch_rk64(int mech, const ReactArgs& a, int nsub_guess, int nsub_min, int nsub_max,
                        hipStream_t stream) {
-  const RK64Opts o{nsub_guess, nsub_min, nsub_max};
-  switch (mech) {
-    case MECH_LIDRYER: return launch_rk64_t<mech_LiDryer>(a, o, stream);
-    case MECH_DRM19: return launch_rk64_t<mech_drm19>(a, o, stream);
-    case MECH_DODECANE53: return launch_rk64_t<mech_dodecane53>(a, o, stream);
-    default: return hipErrorInvalidValue;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->launch_rk64(a, nsub_guess, nsub_min, nsub_max, stream) : hipErrorInvalidValue;
 }
-
 hipError_t launch_cost_keys(int mech, const ReactArgs& a, int mode, unsigned int* keys, int* vals,
                             hipStream_t stream) {
-  switch (mech) {
-    case MECH_LIDRYER: return launch_cost_keys_t<mech_LiDryer>(a, mode, keys, vals, stream);
-    case MECH_DRM19: return launch_cost_keys_t<mech_drm19>(a, mode, keys, vals, stream);
-    case MECH_DODECANE53: return launch_cost_keys_t<mech_dodecane53>(a, mode, keys, vals, stream);
-    default: return hipErrorInvalidValue;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->launch_cost_keys(a, mode, keys, vals, stream) : hipErrorInvalidValue;
 }
-
 hipError_t launch_rhoh_to_temp(int mech, const ArrView& rho, const ArrView& rhoY, const ArrView& rhoH,
                                const ArrView& T, int nx, int ny, long ncell, hipStream_t stream) {
-  switch (mech) {
-    case MECH_LIDRYER: return launch_rhoh_to_temp_t<mech_LiDryer>(rho, rhoY, rhoH, T, nx, ny, ncell, stream);
-    case MECH_DRM19: return launch_rhoh_to_temp_t<mech_drm19>(rho, rhoY, rhoH, T, nx, ny, ncell, stream);
-    case MECH_DODECANE53: return launch_rhoh_to_temp_t<mech_dodecane53>(rho, rhoY, rhoH, T, nx, ny, ncell, stream);
-    default: return hipErrorInvalidValue;
-  }
+  const MechTable* t = mech_table(mech);
+  return t ? t->launch_rhoh_to_temp(rho, rhoY, rhoH, T, nx, ny, ncell, stream) : hipErrorInvalidValue;
 }
 
 // PeleLM::advance_chemistry's per-box glue (Source/PeleLM.cpp:5736-5746, 5764-5771,
@@ -3603,5 +3809,6 @@ hipError_t launch_null(int nspec, const ReactArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL(null_kernel, dim3((unsigned)nb), dim3(256), 0, stream, a, nspec);
   return hipGetLastError();
 }
+#endif   // !PELE_TU_MECH
 
 }  // namespace pelelm

@@ -8,7 +8,7 @@
 
 namespace pelelm {
 
-constexpr int NMAX1 = 64;   // largest ODE size (species + 1) a kernel argument block carries
+constexpr int NMAX1 = 96;   // largest ODE size (species + 1) a kernel argument block carries
 
 // One strided device array over a box: element (cell, n) with cell -> (i, j, k),
 // i fastest, lives at p[(i + ox) + (j + oy) * js + (k + oz) * ks + n * ns].
@@ -82,7 +82,12 @@ enum CounterIndex : int {
 };
 
 // Mechanism ids (order of the dispatch table in reactor_bdf.hip).
-enum MechId : int { MECH_LIDRYER = 0, MECH_DRM19 = 1, MECH_DODECANE53 = 2, NMECH = 3 };
+enum MechId : int {
+  MECH_LIDRYER = 0, MECH_DRM19 = 1, MECH_DODECANE53 = 2, MECH_DODECANE56 = 3, MECH_HEPTANE88 = 4, NMECH = 5
+};
+// the largest ODE size whose linear algebra runs cooperatively (one matrix row per lane
+// of a wave); larger mechanisms factor and solve one cell per lane (mi.lu_form = lane)
+constexpr int COOP_MAX_N1 = 64;
 
 int mech_nspec(int mech);
 const char* mech_name(int mech);
@@ -170,6 +175,29 @@ hipError_t launch_rk64(int mech, const ReactArgs& a, int nsub_guess, int nsub_mi
 // copy a call's rY [nspec][ncell], T and rE [ncell] into flat device arrays
 hipError_t launch_rhoh_to_temp(int mech, const ArrView& rho, const ArrView& rhoY, const ArrView& rhoH,
                                const ArrView& T, int nx, int ny, long ncell, hipStream_t stream);
+
+// One mechanism's entry table.  Each mechanism's kernels compile in a translation unit of
+// their own (reactor_bdf.hip built with PELE_TU_MECH); the dispatch functions above go
+// through these tables.
+struct MechTable {
+  int nspec;
+  const char* name;
+  long ws_slots, iws_slots, mat_stride;
+  hipError_t (*run_react)(const ReactArgs&, hipStream_t, long, long*, KernelProf*, const MlpDev*, TripPlan);
+  hipError_t (*run_react_persist)(const ReactArgs&, const PersistPlan&, long, hipStream_t);
+  int (*persist_capacity)(int);
+  hipError_t (*launch_rates)(const RatesArgs&, hipStream_t);
+  hipError_t (*launch_rk64)(const ReactArgs&, int, int, int, hipStream_t);
+  hipError_t (*launch_cost_keys)(const ReactArgs&, int, unsigned int*, int*, hipStream_t);
+  hipError_t (*launch_rhoh_to_temp)(const ArrView&, const ArrView&, const ArrView&, const ArrView&, int, int, long,
+                                    hipStream_t);
+};
+const MechTable* mech_table(int mech);
+const MechTable* mech_table_LiDryer();
+const MechTable* mech_table_drm19();
+const MechTable* mech_table_dodecane53();
+const MechTable* mech_table_dodecane56();
+const MechTable* mech_table_heptane88();
 hipError_t launch_glue_in(const ArrView& sold, const ArrView& snew, const ArrView& force, double* fcgs, int nspec,
                           int nx, int ny, long ncell, hipStream_t stream);
 hipError_t launch_glue_out(const ArrView& sold, const ArrView& snew, const ArrView& force, const ArrView& rdot,

@@ -72,7 +72,7 @@ def write_table(mech: M.Mechanism, path: str) -> None:
 
 def main(argv=None) -> int:
     from . import hipgen
-    names = argv or ["LiDryer", "drm19", "dodecane53"]
+    names = argv or ["LiDryer", "drm19", "dodecane53", "dodecane56", "heptane88"]
     here = M.HERE
     incdir = os.path.join(os.path.dirname(here), "csrc", "mech")
     os.makedirs(incdir, exist_ok=True)

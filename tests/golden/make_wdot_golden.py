@@ -84,7 +84,7 @@ def wdot(mech, C, T):
 def main():
     out = {}
     only = set(sys.argv[1:])
-    for name, nstate in (("LiDryer", 12), ("drm19", 12), ("dodecane53", 12)):
+    for name, nstate in (("LiDryer", 12), ("drm19", 12), ("dodecane53", 12), ("dodecane56", 8), ("heptane88", 8)):
         if only and name not in only:
             continue
         mech = M.load(name)

@@ -48,7 +48,8 @@ def _parity(label, gY, gT, gFC, ref, tol=None, y_frac=1e-2, dT=1e-7, fc=0.99):
 
 
 @pytest.mark.parametrize("wl,mech", [("lidryer_16", "LiDryer"), ("drm19_64", "drm19"),
-                                     ("dodecane53_64_mix", "dodecane53")])
+                                     ("dodecane53_64_mix", "dodecane53"), ("dodecane56_64_mix", "dodecane56"),
+                                     ("heptane88_64_mix", "heptane88")])
 def test_rates_match_oracle(wl, mech):
     d = cells.generate_workload(wl, ncell=2048)
     m = O.Mech(mech)
@@ -125,9 +126,12 @@ def test_reaction_rates_box_in_pelelm_units(wl, mech):
 
 
 @pytest.mark.parametrize("wl,mech,n", [("lidryer_16", "LiDryer", 4096), ("drm19_64", "drm19", 4096),
-                                       ("drm19_128_mix", "drm19", 4096), ("dodecane53_64_mix", "dodecane53", 2048)])
+                                       ("drm19_128_mix", "drm19", 4096), ("dodecane53_64_mix", "dodecane53", 2048),
+                                       ("dodecane56_64_mix", "dodecane56", 2048), ("heptane88_64_mix", "heptane88", 1024)])
 @pytest.mark.parametrize("schedule", ["plain", "queue"])
 def test_react_matches_oracle(wl, mech, n, schedule):
+    """The reactor against the oracle on every mechanism; the 88-species one factors and
+    solves one cell per lane (its 89 unknowns exceed a wave's rows)."""
     d = cells.generate_workload(wl, ncell=n)
     m = O.Mech(mech)
     ref = m.react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10, stats=True)

@@ -45,7 +45,8 @@ def emu():
 
 
 @pytest.mark.parametrize("wl,mech,n", [("lidryer_16", "LiDryer", 256), ("drm19_64", "drm19", 256),
-                                       ("drm19_128_mix", "drm19", 256), ("dodecane53_64_mix", "dodecane53", 128)])
+                                       ("drm19_128_mix", "drm19", 256), ("dodecane53_64_mix", "dodecane53", 128),
+                                       ("dodecane56_64_mix", "dodecane56", 64), ("heptane88_64_mix", "heptane88", 64)])
 @pytest.mark.parametrize("compact", [1, 0])
 def test_emulated_kernels_match_oracle(emu, wl, mech, n, compact):
     d = cells.generate_workload(wl, ncell=n)

@@ -8,8 +8,10 @@ far fields have equal mixture enthalpy).  Those profiles are copied here as smal
 fixtures (first and last rows only) so the tests run without the reference tree.
 
 The 53-species n-dodecane mechanism keeps the species and order of
-Exec/RegTests/FlameSheet/dodecaneLu_pmf.dat:1; its profile pins the molecular weights and,
-through the far-field balance, the fuel's heat of formation against the core thermo.
+Exec/RegTests/FlameSheet/dodecaneLu_pmf.dat:1, the 56-species one those of
+dodecaneWang_pmf.dat:1 and the 88-species n-heptane one those of heptaneLu_pmf.dat:1;
+their profiles pin the molecular weights and, through the far-field balance, the fuel's
+heat of formation against the core thermo.
 """
 import math
 import os
@@ -19,6 +21,7 @@ import pytest
 from scipy.optimize import brentq
 
 from pelelm_amd.mechanisms import mechanism as M
+from pelelm_amd.mechanisms.make_class_mech import DODECANE56, HEPTANE88
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -31,7 +34,9 @@ PMF_ORDER = {
                    "PXC6H13 C7H14 PXC7H15 C8H16 PXC8H17 C9H18 PXC9H19 C10H20 PXC10H21 PXC12H25 SXC12H25 "
                    "S3XC12H25 C12H24 C12H25O2 C12OOH O2C12H24OOH OC12H23OOH N2").split(),
 }
-MECHS = ["drm19", "LiDryer", "dodecane53"]
+PMF_ORDER["heptane88"] = list(HEPTANE88)     # heptaneLu_pmf.dat:1
+PMF_ORDER["dodecane56"] = list(DODECANE56)   # dodecaneWang_pmf.dat:1
+MECHS = ["drm19", "LiDryer", "dodecane53", "dodecane56", "heptane88"]
 
 # heats of formation at 298.15 K [kJ/mol] (Active Thermochemical Tables / JANAF) and
 # the tolerance each is held to
@@ -40,14 +45,16 @@ DHF = {"H2": (0.0, 0.5), "O2": (0.0, 0.5), "N2": (0.0, 0.5), "AR": (0.0, 0.5), "
        "CH4": (-74.6, 1.0), "CO": (-110.5, 1.0), "CO2": (-393.5, 1.0), "CH3": (146.6, 1.5), "CH2": (391.2, 2.0),
        "CH2(S)": (429.0, 2.0), "HCO": (41.6, 1.5), "CH2O": (-108.9, 1.5), "CH3O": (17.0, 6.0),
        "C2H4": (52.4, 1.0), "C2H5": (120.0, 2.0), "C2H6": (-84.0, 1.0), "CH2D": (429.0, 2.0),
-       "NC12H26": (-290.0, 3.0), "C2H2": (228.2, 2.0), "C2H3": (296.9, 4.0)}
+       "NC12H26": (-290.0, 3.0), "C2H2": (228.2, 2.0), "C2H3": (296.9, 4.0), "NC7H16": (-187.8, 3.0),
+       "CH2CO": (-47.7, 3.0), "CH3CHO": (-166.1, 3.0), "CH3OH": (-201.0, 3.0), "C4H6": (110.0, 3.0)}
 
 
 def fixture_rows(name):
     return np.load(os.path.join(HERE, "golden", f"pmf_endpoints_{name}.npz"))
 
 
-@pytest.mark.parametrize("name,nr", [("drm19", 84), ("LiDryer", 21), ("dodecane53", 180)])
+@pytest.mark.parametrize("name,nr", [("drm19", 84), ("LiDryer", 21), ("dodecane53", 180), ("dodecane56", 340),
+                                     ("heptane88", 434)])
 def test_load_and_order(name, nr):
     m = M.load(name)
     assert m.names == PMF_ORDER[name]
@@ -103,22 +110,25 @@ def test_pmf_ideal_gas_density(name):
         assert abs(M.P_ATM * Wbar / (M.R_CGS * T) / rho - 1.0) < 1e-7
 
 
-def test_pmf_ideal_gas_density_dodecane53():
-    """The dodecane profile was written with a gas constant (or pressure) 1.56e-5 off the
-    CHEMKIN one, the same factor on every row; its two far fields (fuel/air and burnt
-    gas, of very different mean molecular weight) must give that same factor to 1e-9."""
-    m = M.load("dodecane53")
-    d = fixture_rows("dodecane53")
+@pytest.mark.parametrize("name,rho_cgs", [("dodecane53", 1.0), ("dodecane56", 1e-3), ("heptane88", 1.0)])
+def test_pmf_ideal_gas_density_c7_c12(name, rho_cgs):
+    """The dodecane and heptane profiles were written with a gas constant (or pressure) up
+    to 1.6e-5 off the CHEMKIN one, the same factor on every row (dodecaneWang_pmf.dat
+    holds rho in kg/m^3); their two far fields (fuel/air and burnt gas, of very different
+    mean molecular weight) must give that same factor to 1e-9."""
+    m = M.load(name)
+    d = fixture_rows(name)
     W = np.array(m.W)
     f = []
     for row in (d["first"], d["last"]):
-        T, rho, X = row[1], row[3], np.maximum(row[4:], 0.0)
+        T, rho, X = row[1], row[3] * rho_cgs, np.maximum(row[4:], 0.0)
         f.append(M.P_ATM * np.dot(X, W) / X.sum() / (M.R_CGS * T) / rho)
     assert abs(f[0] / f[1] - 1.0) < 1e-9
     assert abs(f[0] - 1.0) < 2e-5
 
 
-@pytest.mark.parametrize("name,tolK", [("drm19", 1.0), ("LiDryer", 0.05), ("dodecane53", 0.5)])
+@pytest.mark.parametrize("name,tolK", [("drm19", 1.0), ("LiDryer", 0.05), ("dodecane53", 0.5), ("dodecane56", 0.5),
+                                       ("heptane88", 0.5)])
 def test_pmf_far_field_enthalpy_balance(name, tolK):
     m = M.load(name)
     d = fixture_rows(name)

@@ -17,7 +17,7 @@ from oracle import oracle as O
 from pelelm_amd import cells
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-MECHS = ["LiDryer", "drm19", "dodecane53"]
+MECHS = ["LiDryer", "drm19", "dodecane53", "dodecane56", "heptane88"]
 
 
 def _random_state(m, rng, Tlo=300.0, Thi=2800.0):
@@ -66,7 +66,8 @@ def test_wdot_conserves_mass_and_elements(name):
 
 
 @pytest.mark.parametrize("name,fuel,o2", [("LiDryer", "H2", 0.5), ("drm19", "CH4", 2.0),
-                                         ("dodecane53", "NC12H26", 18.5)])
+                                         ("dodecane53", "NC12H26", 18.5), ("dodecane56", "NC12H26", 18.5),
+                                         ("heptane88", "NC7H16", 11.0)])
 @pytest.mark.parametrize("phi", [0.6, 1.0, 1.4])
 def test_detailed_balance_at_hp_equilibrium(name, fuel, o2, phi):
     """At the HP-equilibrium state (an independent element-potential solver) every

@@ -35,7 +35,8 @@ def simt():
     stale = not os.path.exists(LIB) or any(os.path.getmtime(s) > os.path.getmtime(LIB) for s in SRC)
     if stale:
         os.makedirs(os.path.dirname(LIB), exist_ok=True)
-        subprocess.run([CLANG, "-O2", "-std=c++17", "-pthread", "-fPIC", "-shared", "-Wno-psabi", "-I" + os.path.join(EMU, "simt"),
+        subprocess.run([CLANG, "-O2", "-std=c++17", "-pthread", "-fPIC", "-shared", "-Wno-psabi", "-DPELE_BASE_MECHS_ONLY",
+                        "-DPELE_WITH_HEPTANE88", "-I" + os.path.join(EMU, "simt"),
                         "-o", LIB, SRC[0]], check=True)
     L = C.CDLL(LIB)
     L.simt_react_trips.restype = C.c_int
@@ -59,10 +60,15 @@ def _check(label, d, ref, rY, T, FC, cnt, n, lu):
     assert cnt[2] == FC.sum()
 
 
-@pytest.mark.parametrize("wl,mech,n", [("drm19_64", "drm19", 128), ("dodecane53_64_mix", "dodecane53", 64)])
+@pytest.mark.parametrize("wl,mech,n", [("drm19_64", "drm19", 128), ("dodecane53_64_mix", "dodecane53", 64),
+                                       ("heptane88_64_mix", "heptane88", 64)])
 @pytest.mark.parametrize("compact,mfma", [(0, 0), (1, 0), (0, 1)])
 def test_cooperative_trip_kernels_match_oracle(simt, wl, mech, n, compact, mfma):
-    """mfma = 1 factors with the blocked LU on (emulated) 16x16x4 matrix-core products."""
+    """mfma = 1 factors with the blocked LU on (emulated) 16x16x4 matrix-core products;
+    the 88-species mechanism factors and solves with a wave per cell, two rows per lane
+    (it has no matrix-core form)."""
+    if mech == "heptane88" and mfma:
+        pytest.skip("above 64 unknowns the factorisation has no matrix-core form")
     simt.simt_set_lu_mfma(mfma)
     before = simt.simt_mfma_count()
     d = cells.generate_workload(wl, ncell=n)
