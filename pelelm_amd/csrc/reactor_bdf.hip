@@ -3081,18 +3081,48 @@ __device__ __forceinline__ void lu_wide_one(const ReactArgs& a, const int* __res
     const bool nonzero = kmax > 1ull;
     if (!nonzero && sing == 0) sing = k + 1;
     if (nonzero) {
+      // both rows of the lane in one pass over the pivot row, eight columns at a time
+      // (the pivot row's words loaded before the rows' stores, so the loads overlap)
       const double inv = 1.0 / sm[prow * LD + k];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int r = h ? r1 : r0;
-        const bool d = h ? done1 : done0;
-        if (d || r == prow) continue;
-        const double lik = sm[r * LD + k] * inv;
-        sm[r * LD + k] = lik;
-        if (lik != 0.0) {
+      double l0 = 0.0, l1 = 0.0;
+      if (!done0 && r0 != prow) {
+        l0 = sm[r0 * LD + k] * inv;
+        sm[r0 * LD + k] = l0;
+      }
+      if (!done1 && r1 != prow) {
+        l1 = sm[r1 * LD + k] * inv;
+        sm[r1 * LD + k] = l1;
+      }
+      const bool u0 = l0 != 0.0, u1 = l1 != 0.0;   // a zero multiplier leaves its row as is
+      const double* pr = &sm[prow * LD];
+      double* a0 = &sm[(u0 ? r0 : 0) * LD];
+      double* a1 = &sm[(u1 ? r1 : 0) * LD];
+      int j = k + 1;
 #pragma unroll 1
-          for (int j = k + 1; j < N1; ++j) sm[r * LD + j] -= lik * sm[prow * LD + j];
+      for (; j + 8 <= N1; j += 8) {
+        double p[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) p[t] = pr[j + t];
+        if (u0) {
+          double v[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) v[t] = a0[j + t];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) a0[j + t] = v[t] - l0 * p[t];
         }
+        if (u1) {
+          double v[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) v[t] = a1[j + t];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) a1[j + t] = v[t] - l1 * p[t];
+        }
+      }
+#pragma unroll 1
+      for (; j < N1; ++j) {
+        const double pj = pr[j];
+        if (u0) a0[j] -= l0 * pj;
+        if (u1) a1[j] -= l1 * pj;
       }
     }
     if (r0 == prow) done0 = true;
@@ -3141,23 +3171,49 @@ __device__ __forceinline__ void solve_wide_one(const ReactArgs& a, const int* __
   const bool has0 = r0 < N1, has1 = r1 < N1;
   double y0 = has0 ? mat[MS::B + mat.perm(MS::PERM, r0)] : 0.0;
   double y1 = has1 ? mat[MS::B + mat.perm(MS::PERM, r1)] : 0.0;
+  // eight columns of the lane's rows are loaded ahead of their eight dependent steps
+  const double* f0 = &mat[MS::LU + (has0 ? r0 : 0) * N1];
+  const double* f1 = &mat[MS::LU + (has1 ? r1 : 0) * N1];
 #pragma unroll 1
-  for (int k = 0; k < N1 - 1; ++k) {
-    const double yk = lane_read(k < 64 ? y0 : y1, k & 63);
-    if (has0 && r0 > k) y0 -= mat[MS::LU + r0 * N1 + k] * yk;
-    if (has1 && r1 > k) y1 -= mat[MS::LU + r1 * N1 + k] * yk;
+  for (int k0 = 0; k0 < N1 - 1; k0 += 8) {
+    double c0[8], c1[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 + t;
+      c0[t] = (has0 && r0 > k && k < N1 - 1) ? f0[k] : 0.0;
+      c1[t] = (has1 && r1 > k && k < N1 - 1) ? f1[k] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 + t;
+      if (k < N1 - 1) {
+        const double yk = lane_read(k < 64 ? y0 : y1, k & 63);
+        if (c0[t] != 0.0) y0 -= c0[t] * yk;
+        if (c1[t] != 0.0) y1 -= c1[t] * yk;
+      }
+    }
   }
 #pragma unroll 1
-  for (int k = N1 - 1; k >= 0; --k) {
-    // the owner divides by the pivot; the quotient goes to every lane
-    if (k < 64) {
-      if (l == k) y0 = y0 / mat[MS::LU + k * N1 + k];
-    } else {
-      if (l == k - 64) y1 = y1 / mat[MS::LU + k * N1 + k];
+  for (int k0 = N1 - 1; k0 >= 0; k0 -= 8) {
+    double c0[8], c1[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 - t;   // the diagonal for the owner, U entries for the rows above
+      c0[t] = (has0 && r0 <= k && k >= 0) ? f0[k] : 0.0;
+      c1[t] = (has1 && r1 <= k && k >= 0) ? f1[k] : 0.0;
     }
-    const double xk = lane_read(k < 64 ? y0 : y1, k & 63);
-    if (has0 && r0 < k) y0 -= mat[MS::LU + r0 * N1 + k] * xk;
-    if (has1 && r1 < k) y1 -= mat[MS::LU + r1 * N1 + k] * xk;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 - t;
+      if (k >= 0) {
+        // the owner divides by the pivot; the quotient goes to every lane
+        if (r0 == k) y0 = y0 / c0[t];
+        if (r1 == k) y1 = y1 / c1[t];
+        const double xk = lane_read(k < 64 ? y0 : y1, k & 63);
+        if (r0 < k && c0[t] != 0.0) y0 -= c0[t] * xk;
+        if (r1 < k && c1[t] != 0.0) y1 -= c1[t] * xk;
+      }
+    }
   }
   if (has0) mat[MS::X + r0] = y0;
   if (has1) mat[MS::X + r1] = y1;
