@@ -131,7 +131,7 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
      (coop: 32 lanes per cell, a whole wave above 32 unknowns, default | lane: one lane per
      cell | mfma: one wave per cell, blocked LU in LDS with the trailing updates on 16x16x4
      FP64 or FP32 matrix-core tiles; the trip schedules only, the persistent kernel factors
-     cooperatively), mi.ir_tol (1e-4: a refinement sweep whose correction has WRMS norm at most this,
+     cooperatively; above 64 unknowns it selects the LU staged in LDS), mi.ir_tol (1e-4: a refinement sweep whose correction has WRMS norm at most this,
      in error-weight units, ends the refinement; a solve that does not get there in
      mi.ir_sweeps sweeps makes its cell refactor in FP64), mi.max_trips (0: 3 x
      cvode.max_num_steps + 100; cells still integrating after this many trips fail),

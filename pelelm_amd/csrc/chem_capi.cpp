@@ -559,8 +559,8 @@ int react_common(Reactor* r, ReactArgs& a, double* dt, double* time, hipStream_t
   const bool wide = N1 > COOP_MAX_N1;
   if (wide && !r->rk64 && !r->null_reactor) {
     // one matrix row per lane of a wave does not reach: one wave per cell with two rows
-    // per lane, FP64 factors (or one lane per cell with mi.lu_form = lane)
-    a.lu_mfma = 0;
+    // per lane, FP64 factors, the rows in registers (mi.lu_form = mfma: staged in LDS;
+    // lane: one lane per cell)
     a.lu_mode = LU_FP64;
     if (fixed || r->schedule == SCHED_PERSIST)
       return fail(r, MI_EINVAL, "mechanisms above 63 species run on the trip schedules (plain, queue, auto), not "

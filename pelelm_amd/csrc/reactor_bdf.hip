@@ -3141,6 +3141,106 @@ __device__ __forceinline__ void lu_wide_one(const ReactArgs& a, const int* __res
   __syncthreads();   // the LDS is reused by the next cell of this block
 }
 
+// The same factorisation with both rows of a lane in registers (the cooperative LU's
+// form, two rows per lane): the column loop unrolled at compile time so the rows stay
+// in registers, the pivot row's entries read from its lane with v_readlane.
+template <class M>
+__device__ __forceinline__ void lu_wide_reg_one(const ReactArgs& a, const int* __restrict__ list, const long n,
+                                                const long idx) {
+  constexpr int N1 = M::N1;
+  using S = Slots<N1>;
+  using MS = MatSlots<N1>;
+  if (idx >= n) return;   // uniform over the block (one cell per block)
+  const long c = list[idx];
+  const WS w = cell_ws<M>(a.ws, c);
+  if (LANE_FIELD(M, w, setup_pending) == 0) return;
+  const double gamma = LANE_FIELD(M, w, gamma);
+  const CM mat = cell_mat<M>(a.mat, c);
+  const int r0 = threadIdx.x, r1 = threadIdx.x + 64;
+  const bool has1 = r1 < N1;
+  double m0[N1], m1[N1];
+#pragma unroll
+  for (int j = 0; j < N1; ++j) {
+    m0[j] = (r0 == j ? 1.0 : 0.0) - gamma * mat[MS::JS + r0 * N1 + j];
+    m1[j] = has1 ? (r1 == j ? 1.0 : 0.0) - gamma * mat[MS::JS + r1 * N1 + j] : 0.0;
+  }
+  bool done0 = false, done1 = !has1;
+  int step0 = 0, step1 = 0, sing = 0;
+  static_for<N1>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    unsigned long long k0 = 0ull, k1 = 0ull;
+    if (!done0) {
+      double av = fabs(m0[k]);
+      if (av != av) av = HUGE_VAL;
+      k0 = (unsigned long long)__double_as_longlong(av) + 1ull;
+    }
+    if (!done1) {
+      double av = fabs(m1[k]);
+      if (av != av) av = HUGE_VAL;
+      k1 = (unsigned long long)__double_as_longlong(av) + 1ull;
+    }
+    unsigned long long kmax = k0 > k1 ? k0 : k1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(kmax, off, 64);
+      kmax = o > kmax ? o : kmax;
+    }
+    const unsigned long long b0 = __ballot(k0 == kmax);
+    const unsigned long long b1 = __ballot(k1 == kmax);
+    const int prow = b0 ? __ffsll((long long)b0) - 1 : 64 + __ffsll((long long)b1) - 1;
+    const bool nonzero = kmax > 1ull;
+    if (!nonzero && sing == 0) sing = k + 1;
+    const bool hi = prow >= 64;
+    const int src = prow & 63;
+    const bool u0 = !done0 && r0 != prow && nonzero, u1 = !done1 && r1 != prow && nonzero;
+    const double pk = lane_read(hi ? m1[k] : m0[k], src);
+    double l0 = 0.0, l1 = 0.0;
+    if (u0) {
+      l0 = m0[k] * (1.0 / pk);
+      m0[k] = l0;
+    }
+    if (u1) {
+      l1 = m1[k] * (1.0 / pk);
+      m1[k] = l1;
+    }
+    const bool v0 = u0 && l0 != 0.0, v1 = u1 && l1 != 0.0;
+#pragma unroll
+    for (int j = k + 1; j < N1; ++j) {
+      const double pj = lane_read(hi ? m1[j] : m0[j], src);
+      if (v0) m0[j] -= l0 * pj;
+      if (v1) m1[j] -= l1 * pj;
+    }
+    if (r0 == prow) {
+      done0 = true;
+      step0 = k;
+    }
+    if (r1 == prow) {
+      done1 = true;
+      step1 = k;
+    }
+  });
+#pragma unroll
+  for (int j = 0; j < N1; ++j) {
+    mat[MS::LU + step0 * N1 + j] = m0[j];
+    if (has1) mat[MS::LU + step1 * N1 + j] = m1[j];
+  }
+  mat.perm(MS::PERM, step0) = r0;
+  if (has1) mat.perm(MS::PERM, step1) = r1;
+  if (threadIdx.x == 0) w[S::SING] = (double)sing;
+}
+
+template <class M, bool STRIDE>
+__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(1) k_lu_wide_reg(const ReactArgs a,
+                                                                         const int* __restrict__ list,
+                                                                         const unsigned int* __restrict__ count) {
+  const long n = (long)*count;
+  if (STRIDE) {
+    for (long blk = xcd_block(); blk < n; blk += gridDim.x) lu_wide_reg_one<M>(a, list, n, blk);
+  } else {
+    lu_wide_reg_one<M>(a, list, n, xcd_block());
+  }
+}
+
 template <class M, bool STRIDE>
 __global__ void __launch_bounds__(64) k_lu_wide(const ReactArgs a, const int* __restrict__ list,
                                                 const unsigned int* __restrict__ count) {
@@ -3244,10 +3344,17 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
   }
   if constexpr (M::N1 > COOP_MAX_N1) {   // one cell per block; a multiple of 8 blocks (XCD mapping)
     const dim3 g(exact ? (unsigned)(n > 0 ? ((n + 7) / 8) * 8 : 8) : 8192u);
-    if (exact)
-      hipLaunchKernelGGL((k_lu_wide<M, false>), g, dim3(64), 0, stream, a, list, count);
-    else
-      hipLaunchKernelGGL((k_lu_wide<M, true>), g, dim3(64), 0, stream, a, list, count);
+    if (a.lu_mfma) {   // the staged-in-LDS form (mi.lu_form = mfma selects it above 64 unknowns)
+      if (exact)
+        hipLaunchKernelGGL((k_lu_wide<M, false>), g, dim3(64), 0, stream, a, list, count);
+      else
+        hipLaunchKernelGGL((k_lu_wide<M, true>), g, dim3(64), 0, stream, a, list, count);
+    } else {
+      if (exact)
+        hipLaunchKernelGGL((k_lu_wide_reg<M, false>), g, dim3(64), 0, stream, a, list, count);
+      else
+        hipLaunchKernelGGL((k_lu_wide_reg<M, true>), g, dim3(64), 0, stream, a, list, count);
+    }
   } else {
   if (a.lu_mfma) {   // one cell per block; a multiple of 8 blocks (XCD mapping)
     const dim3 g(exact ? (unsigned)(n > 0 ? ((n + 7) / 8) * 8 : 8) : 8192u);

@@ -491,6 +491,25 @@ def test_gmres_rejects_what_it_cannot_run():
         r.close()
 
 
+@pytest.mark.parametrize("lu_form", ["mfma", "lane"])
+def test_88_species_factorisation_forms_match_oracle(lu_form):
+    """Above 64 unknowns: the LU staged in LDS (mi.lu_form = mfma selects it there) and the
+    one-lane LU and solves, against the oracle (the default keeps the rows in registers,
+    covered by test_react_matches_oracle)."""
+    n = 512
+    d = cells.generate_workload("heptane88_64_mix", ncell=n)
+    ref = O.Mech("heptane88").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    r = _mk("heptane88", ode__rtol=1e-6, ode__atol=1e-10, mi__lu_form=lu_form)
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    assert st["nfail"] == 0 and st["ncells"] == n
+    _parity(f"heptane88 lu_form={lu_form}", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)
+
+
 def test_one_lane_solves_in_graph_trips_cover_every_cell():
     """mi.lu_form = lane with graph-captured trips on more cells than one fixed solve grid
     of 16,384 lanes holds: the one-lane solve's grid is sized for every cell, so cells
