@@ -311,7 +311,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="drm19_128_mix")
     ap.add_argument("--ncell", type=int, default=None, help="override the cell count (default: the workload's)")
     ap.add_argument("--forms", default="plain,plain+unordered,plain+fc,plain+fp32,plain+fp32+ir0,queue+fp32+pred,"
-                                       "persistent")
+                                       "persistent,plain+mfma,plain+gmres")
     ap.add_argument("--pred-pairs", type=int, default=1_000_000, help="training pairs for +pred forms")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample duration")
     ap.add_argument("--tight-cells", type=int, default=4096, help="sampled cells for the tight-oracle errors")
