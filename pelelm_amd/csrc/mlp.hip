@@ -1,6 +1,6 @@
 // Learned Newton predictor: a 4-layer perceptron (n_in -> 128 -> 128 -> 128 -> n_out,
 // GELU on the hidden layers) with BF16 weights and activations multiplied on the
-// matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulation), and its training:
+// matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation), and its training:
 // forward with stored pre-activations, backward (MSE loss), gradient reduction and
 // Adam on FP32 master weights.
 //
@@ -146,15 +146,15 @@ __global__ void __launch_bounds__(THREADS) k_mlp_bwd(const MlpDev w, const float
       const int nt = t / (KP / 16), kt = t % (KP / 16);
       v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int rs = 0; rs < ROWS / 16; ++rs) {
-        v4s a, b;
+      for (int rs = 0; rs < ROWS / 32; ++rs) {
+        unsigned short a[8], b[8];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int r = rs * 16 + 4 * lq + i;
-          a[i] = (short)g[r * LDW + nt * 16 + lr];
-          b[i] = (short)hin[r * LDW + kt * 16 + lr];
+        for (int i = 0; i < 8; ++i) {
+          const int r = rs * 32 + 8 * lq + i;
+          a[i] = g[r * LDW + nt * 16 + lr];
+          b[i] = hin[r * LDW + kt * 16 + lr];
         }
-        acc = mlp_mfma(a, b, acc);
+        acc = mlp_mfma(mlp_pack8(a), mlp_pack8(b), acc);
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -174,9 +174,9 @@ __global__ void __launch_bounds__(THREADS) k_mlp_bwd(const MlpDev w, const float
 #pragma unroll 1
     for (int ct = 0; ct < MLP_HIDDEN / 16; ++ct) {
       v4f acc = {0.f, 0.f, 0.f, 0.f};
-      for (int kt = 0; kt < NP / 16; ++kt) {
-        const v4s a = *reinterpret_cast<const v4s*>(&g[(wave * 16 + lr) * LDW + kt * 16 + 4 * lq]);
-        const v4s b = *reinterpret_cast<const v4s*>(&wk[(long)(ct * 16 + lr) * NP + kt * 16 + 4 * lq]);
+      for (int kt = 0; kt < NP / 32; ++kt) {
+        const bf16x8 a = mlp_ld8(&g[(wave * 16 + lr) * LDW + kt * 32 + 8 * lq]);
+        const bf16x8 b = mlp_ld8(&wk[(long)(ct * 16 + lr) * NP + kt * 32 + 8 * lq]);
         acc = mlp_mfma(a, b, acc);
       }
 #pragma unroll

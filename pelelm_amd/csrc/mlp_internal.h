@@ -39,16 +39,18 @@ struct MlpDev {
 // ---------------------------------------------------------------------------
 // device building blocks shared by the trainer (mlp.hip) and the reactor's predictor
 // kernel.  A workgroup of MLP_THREADS (four waves) owns MLP_ROWS samples.
-// MFMA 16x16x16 bf16 operand layout (lane l of 64, i = 0..3):
-//   A (16 x 16, M x K): A[l % 16][4 (l / 16) + i]
-//   B (16 x 16, K x N): B[4 (l / 16) + i][l % 16]
-//   D (16 x 16, M x N): D[4 (l / 16) + i][l % 16]
+// gfx950's MFMA 16x16x32 bf16 (v_mfma_f32_16x16x32_bf16, twice the K of CDNA3's
+// 16x16x16 form per instruction) operand layout, lane l of 64 (checked by a GPU unit
+// test on raw registers):
+//   A (16 x 32, M x K): A[l % 16][8 (l / 16) + i], i = 0..7
+//   B (32 x 16, K x N): B[8 (l / 16) + i][l % 16], i = 0..7
+//   D (16 x 16, M x N): D[4 (l / 16) + v][l % 16], v = 0..3
 // ---------------------------------------------------------------------------
 constexpr int MLP_ROWS = 64;              // samples per workgroup
 constexpr int MLP_THREADS = 256;          // four waves, 16 rows each in the row-tiled products
 constexpr int MLP_LDW = MLP_HIDDEN + 8;   // LDS row stride of an activation buffer (bf16)
 
-typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float mlp_gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
@@ -56,8 +58,20 @@ __device__ __forceinline__ float mlp_gelu_d(float z) {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-__device__ __forceinline__ v4f mlp_mfma(v4s a, v4s b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+__device__ __forceinline__ v4f mlp_mfma(bf16x8 a, bf16x8 b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+// eight consecutive bf16 words (16-byte aligned) as one operand
+__device__ __forceinline__ bf16x8 mlp_ld8(const unsigned short* p) {
+  bf16x8 v;
+  __builtin_memcpy(&v, p, sizeof(v));
+  return v;
+}
+// eight bf16 words gathered one by one
+__device__ __forceinline__ bf16x8 mlp_pack8(const unsigned short (&w)[8]) {
+  bf16x8 v;
+  __builtin_memcpy(&v, w, sizeof(v));
+  return v;
 }
 
 // one dense layer on the 64 rows of this workgroup: Hin [64][KP] (LDS) x W -> NP outputs;
@@ -72,9 +86,9 @@ __device__ __forceinline__ void mlp_dense(const unsigned short* Hin, const unsig
   for (int nt = 0; nt < NP / 16; ++nt) {
     v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < KP / 16; ++kt) {
-      const v4s a = *reinterpret_cast<const v4s*>(&Hin[(m0 + lr) * MLP_LDW + kt * 16 + 4 * lq]);
-      const v4s b = *reinterpret_cast<const v4s*>(&wt[(long)(nt * 16 + lr) * KP + kt * 16 + 4 * lq]);
+    for (int kt = 0; kt < KP / 32; ++kt) {
+      const bf16x8 a = mlp_ld8(&Hin[(m0 + lr) * MLP_LDW + kt * 32 + 8 * lq]);
+      const bf16x8 b = mlp_ld8(&wt[(long)(nt * 16 + lr) * KP + kt * 32 + 8 * lq]);
       acc = mlp_mfma(a, b, acc);
     }
     const int col = nt * 16 + lr;

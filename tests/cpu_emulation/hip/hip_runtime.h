@@ -67,7 +67,7 @@ template <class T> inline T __shfl_xor(T v, int, int = 64) { return v; }
 inline void __syncthreads() {}
 // matrix-core products need all 64 lanes at once: kernels that use them (the learned
 // predictor) are compiled here but never launched
-#define __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, x, y, z) (c)
+#define __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, x, y, z) (c)
 // one lane at a time cannot run a matrix-core product (mi.lu_form = mfma is not launched here)
 #define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) (c)
 #define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) (c)

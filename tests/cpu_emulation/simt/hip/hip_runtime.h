@@ -321,7 +321,7 @@ inline int simt_readlane(int v, int l) {
 #define __builtin_amdgcn_readlane(v, l) simt_readlane((v), (l))
 // the learned predictor's matrix-core products are not emulated (its kernels are not
 // launched here)
-#define __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, x, y, z) (c)
+#define __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, x, y, z) (c)
 // v_mfma_{f64,f32}_16x16x4: D = C + A B over one 16 x 16 x 4 block, with the operand
 // layout measured on gfx950: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; its
 // register v holds D[l / 16 + 4 v][l % 16] (FP64) or D[4 (l / 16) + v][l % 16] (FP32).

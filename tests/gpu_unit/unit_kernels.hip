@@ -348,6 +348,23 @@ __global__ void __launch_bounds__(64) k_mfma_probe(const double* a, const double
   cv = MatCore<Real>::mac((Real)a[l], (Real)b[l], cv);
   for (int v = 0; v < 4; ++v) d[4 * l + v] = (double)cv[v];
 }
+
+// one v_mfma_f32_16x16x32_bf16 on raw lane registers: lane l passes a[8 l + i], b[8 l + i]
+// (i = 0..7, values exactly representable in bf16) and c[4 l + v], and returns d[4 l + v]
+typedef __bf16 unit_bf16x8 __attribute__((ext_vector_type(8)));
+__global__ void __launch_bounds__(64) k_mfma_bf16_probe(const float* a, const float* b, const float* c, float* d) {
+  const int l = threadIdx.x;
+  unit_bf16x8 av, bv;
+  for (int i = 0; i < 8; ++i) {
+    av[i] = (__bf16)a[8 * l + i];
+    bv[i] = (__bf16)b[8 * l + i];
+  }
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  v4 cv;
+  for (int v = 0; v < 4; ++v) cv[v] = c[4 * l + v];
+  cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, cv, 0, 0, 0);
+  for (int v = 0; v < 4; ++v) d[4 * l + v] = cv[v];
+}
 #endif
 
 template <class M>
@@ -457,6 +474,22 @@ extern "C" int unit_mfma_probe(int f64, const double* A, const double* B, const 
   UNIT_SYNC();
   UNIT_D2H(D, dD, 256 * sizeof(double));
   for (double* p : {dA, dB, dC, dD}) UNIT_DEV_FREE(p);
+  return e == hipSuccess ? 0 : -3;
+#endif
+}
+
+// a, b: 512 lane values (eight per lane), c, d: 256 (four per lane)
+extern "C" int unit_mfma_bf16_probe(const float* A, const float* B, const float* Cm, float* D) {
+#ifdef UNIT_HOST
+  (void)A; (void)B; (void)Cm; (void)D;
+  return -2;
+#else
+  float *dA = dev_copy(A, 512), *dB = dev_copy(B, 512), *dC = dev_copy(Cm, 256), *dD = dev_zero<float>(256);
+  hipLaunchKernelGGL(k_mfma_bf16_probe, dim3(1), dim3(64), 0, nullptr, dA, dB, dC, dD);
+  const hipError_t e = hipGetLastError();
+  UNIT_SYNC();
+  UNIT_D2H(D, dD, 256 * sizeof(float));
+  for (float* p : {dA, dB, dC, dD}) UNIT_DEV_FREE(p);
   return e == hipSuccess ? 0 : -3;
 #endif
 }

@@ -390,6 +390,32 @@ def test_matrix_core_operand_layout(f64):
 
 
 @pytest.mark.gpu
+def test_bf16_matrix_core_operand_layout():
+    """One v_mfma_f32_16x16x32_bf16 (the learned predictor's product) on raw lane
+    registers with the layout the network kernels assume: lane l holds A[l % 16][8 (l / 16)
+    + i] and B[8 (l / 16) + i][l % 16] (i = 0..7) and D[4 (l / 16) + v][l % 16].  Small
+    integer quarters are exact in bf16 and their sums exact in FP32, so D must equal
+    C + A B bit for bit; transposed operands fail."""
+    G = _load("libunit_gpu.so")
+    rng = np.random.default_rng(8)
+    A = rng.integers(-8, 9, size=(16, 32)) / 4.0
+    B = rng.integers(-8, 9, size=(32, 16)) / 4.0
+    Cm = rng.integers(-8, 9, size=(16, 16)) / 4.0
+    lane = np.arange(64)
+    a = np.array([A[l % 16, 8 * (l // 16) + i] for l in lane for i in range(8)], dtype=np.float32)
+    b = np.array([B[8 * (l // 16) + i, l % 16] for l in lane for i in range(8)], dtype=np.float32)
+    rows = np.array([4 * (l // 16) + v for l in lane for v in range(4)])
+    cols = np.array([l % 16 for l in lane for v in range(4)])
+    c = np.ascontiguousarray(Cm[rows, cols], dtype=np.float32)
+    d = np.zeros(256, dtype=np.float32)
+    assert G.unit_mfma_bf16_probe(_p(a), _p(b), _p(c), _p(d)) == 0
+    D = np.full((16, 16), np.nan)
+    D[rows, cols] = d
+    np.testing.assert_array_equal(D, Cm + A @ B)
+    assert np.abs(D - (Cm + A @ B[::-1])).max() > 0.1   # the check can fail
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mech,wl", CASES)
 @pytest.mark.parametrize("fp32", [0, 1])
 def test_matrix_core_lu_matches_cooperative_lu(mech, wl, fp32):
