@@ -343,6 +343,23 @@ def test_auto_schedule_tunes_once_per_box_size_and_caches(wl, mech, n, tmp_path)
     assert mech in txt and str(n) in txt and '"schedule"' in txt
 
 
+def test_auto_schedule_on_the_88_species_mechanism():
+    """mi.schedule = auto above 64 unknowns searches the trip schedules only (5 candidates:
+    the persistent kernel does not run these mechanisms) and matches the oracle."""
+    n = 1024
+    d = cells.generate_workload("heptane88_64_mix", ncell=n)
+    ref = O.Mech("heptane88").react(d["rY"], d["T"], d["rE"], d["dt"], rtol=1e-6, atol=1e-10)
+    r = _mk("heptane88", ode__rtol=1e-6, ode__atol=1e-10, mi__schedule="auto")
+    rY, T, rE = _dev(d["rY"]), _dev(d["T"]), _dev(d["rE"])
+    FC = torch.zeros(n, dtype=torch.float64, device=DEV)
+    r.react(rY, None, T, rE, None, FC, None, d["dt"], 0.0)
+    torch.cuda.synchronize()
+    st = r.stats()
+    r.close()
+    assert st["nfail"] == 0 and st["tune_ms"][2] == 5 and st["schedule_used"] in (0, 1)
+    _parity("heptane88 auto", rY.cpu().numpy(), T.cpu().numpy(), FC.cpu().numpy(), ref)
+
+
 def test_typical_values_scale_the_absolute_tolerances():
     """peleLM.use_typ_vals_chem = 1 with PeleLM's typical values
     (update_typical_values_chem, Source/PeleLM.cpp:1497-1516: max(typical_Y_val_min,
