@@ -3141,105 +3141,98 @@ __device__ __forceinline__ void lu_wide_one(const ReactArgs& a, const int* __res
   __syncthreads();   // the LDS is reused by the next cell of this block
 }
 
-// The same factorisation with both rows of a lane in registers (the cooperative LU's
-// form, two rows per lane): the column loop unrolled at compile time so the rows stay
-// in registers, the pivot row's entries read from its lane with v_readlane.
+// The same factorisation on two waves per cell, one row per thread (thread t holds row
+// t in registers, the column loop unrolled at compile time): each step's pivot from the
+// two waves' 64-bit maxima exchanged through LDS (ties to the lower row), the pivot row
+// broadcast through LDS, both double-buffered by step parity so one barrier pair per
+// column suffices.
 template <class M>
-__device__ __forceinline__ void lu_wide_reg_one(const ReactArgs& a, const int* __restrict__ list, const long n,
-                                                const long idx) {
+__device__ __forceinline__ void lu_wide2_one(const ReactArgs& a, const int* __restrict__ list, const long n,
+                                             const long idx) {
   constexpr int N1 = M::N1;
   using S = Slots<N1>;
   using MS = MatSlots<N1>;
+  static_assert(N1 <= 128, "one row per thread of two waves");
+  __shared__ double prow_buf[2][N1];
+  __shared__ unsigned long long wkey[2][2];
+  __shared__ int wwho[2][2];
   if (idx >= n) return;   // uniform over the block (one cell per block)
   const long c = list[idx];
   const WS w = cell_ws<M>(a.ws, c);
   if (LANE_FIELD(M, w, setup_pending) == 0) return;
   const double gamma = LANE_FIELD(M, w, gamma);
   const CM mat = cell_mat<M>(a.mat, c);
-  const int r0 = threadIdx.x, r1 = threadIdx.x + 64;
-  const bool has1 = r1 < N1;
-  double m0[N1], m1[N1];
+  const int t = threadIdx.x, wave = t >> 6;
+  const bool row = t < N1;
+  double m[N1];
 #pragma unroll
-  for (int j = 0; j < N1; ++j) {
-    m0[j] = (r0 == j ? 1.0 : 0.0) - gamma * mat[MS::JS + r0 * N1 + j];
-    m1[j] = has1 ? (r1 == j ? 1.0 : 0.0) - gamma * mat[MS::JS + r1 * N1 + j] : 0.0;
-  }
-  bool done0 = false, done1 = !has1;
-  int step0 = 0, step1 = 0, sing = 0;
+  for (int j = 0; j < N1; ++j) m[j] = row ? (t == j ? 1.0 : 0.0) - gamma * mat[MS::JS + t * N1 + j] : 0.0;
+  bool done = !row;
+  int my_step = 0, sing = 0;
   static_for<N1>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    unsigned long long k0 = 0ull, k1 = 0ull;
-    if (!done0) {
-      double av = fabs(m0[k]);
+    constexpr int par = k & 1;
+    unsigned long long key = 0ull;
+    if (!done) {
+      double av = fabs(m[k]);
       if (av != av) av = HUGE_VAL;
-      k0 = (unsigned long long)__double_as_longlong(av) + 1ull;
+      key = (unsigned long long)__double_as_longlong(av) + 1ull;
     }
-    if (!done1) {
-      double av = fabs(m1[k]);
-      if (av != av) av = HUGE_VAL;
-      k1 = (unsigned long long)__double_as_longlong(av) + 1ull;
-    }
-    unsigned long long kmax = k0 > k1 ? k0 : k1;
+    unsigned long long kmax = key;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       const unsigned long long o = __shfl_xor(kmax, off, 64);
       kmax = o > kmax ? o : kmax;
     }
-    const unsigned long long b0 = __ballot(k0 == kmax);
-    const unsigned long long b1 = __ballot(k1 == kmax);
-    const int prow = b0 ? __ffsll((long long)b0) - 1 : 64 + __ffsll((long long)b1) - 1;
-    const bool nonzero = kmax > 1ull;
+    const unsigned long long ball = __ballot(key == kmax);
+    if ((t & 63) == 0) {
+      wkey[par][wave] = kmax;
+      wwho[par][wave] = wave * 64 + __ffsll((long long)ball) - 1;
+    }
+    __syncthreads();
+    const unsigned long long k0 = wkey[par][0], k1 = wkey[par][1];
+    const int prow = (k0 >= k1) ? wwho[par][0] : wwho[par][1];   // rows of wave 0 are the lower ones
+    const unsigned long long gmax = k0 >= k1 ? k0 : k1;
+    const bool nonzero = gmax > 1ull;
     if (!nonzero && sing == 0) sing = k + 1;
-    const bool hi = prow >= 64;
-    const int src = prow & 63;
-    const bool u0 = !done0 && r0 != prow && nonzero, u1 = !done1 && r1 != prow && nonzero;
-    const double pk = lane_read(hi ? m1[k] : m0[k], src);
-    double l0 = 0.0, l1 = 0.0;
-    if (u0) {
-      l0 = m0[k] * (1.0 / pk);
-      m0[k] = l0;
-    }
-    if (u1) {
-      l1 = m1[k] * (1.0 / pk);
-      m1[k] = l1;
-    }
-    const bool v0 = u0 && l0 != 0.0, v1 = u1 && l1 != 0.0;
+    if (t == prow) {
 #pragma unroll
-    for (int j = k + 1; j < N1; ++j) {
-      const double pj = lane_read(hi ? m1[j] : m0[j], src);
-      if (v0) m0[j] -= l0 * pj;
-      if (v1) m1[j] -= l1 * pj;
+      for (int j = k; j < N1; ++j) prow_buf[par][j] = m[j];
     }
-    if (r0 == prow) {
-      done0 = true;
-      step0 = k;
+    __syncthreads();
+    if (!done && t != prow && nonzero) {
+      const double l = m[k] * (1.0 / prow_buf[par][k]);
+      m[k] = l;
+      if (l != 0.0) {
+#pragma unroll
+        for (int j = k + 1; j < N1; ++j) m[j] -= l * prow_buf[par][j];
+      }
     }
-    if (r1 == prow) {
-      done1 = true;
-      step1 = k;
+    if (t == prow && !done) {
+      done = true;
+      my_step = k;
     }
   });
+  if (row) {
 #pragma unroll
-  for (int j = 0; j < N1; ++j) {
-    mat[MS::LU + step0 * N1 + j] = m0[j];
-    if (has1) mat[MS::LU + step1 * N1 + j] = m1[j];
+    for (int j = 0; j < N1; ++j) mat[MS::LU + my_step * N1 + j] = m[j];
+    mat.perm(MS::PERM, my_step) = t;
   }
-  mat.perm(MS::PERM, step0) = r0;
-  if (has1) mat.perm(MS::PERM, step1) = r1;
-  if (threadIdx.x == 0) w[S::SING] = (double)sing;
+  if (t == 0) w[S::SING] = (double)sing;
+  __syncthreads();   // the LDS is reused by the next cell of this block
 }
 
 template <class M, bool STRIDE>
-__global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(1) k_lu_wide_reg(const ReactArgs a,
-                                                                         const int* __restrict__ list,
-                                                                         const unsigned int* __restrict__ count) {
+__global__ void __launch_bounds__(128) k_lu_wide2(const ReactArgs a, const int* __restrict__ list,
+                                                  const unsigned int* __restrict__ count) {
   const long n = (long)*count;
   if (STRIDE) {
-    for (long blk = xcd_block(); blk < n; blk += gridDim.x) lu_wide_reg_one<M>(a, list, n, blk);
+    for (long blk = xcd_block(); blk < n; blk += gridDim.x) lu_wide2_one<M>(a, list, n, blk);
   } else {
-    lu_wide_reg_one<M>(a, list, n, xcd_block());
+    lu_wide2_one<M>(a, list, n, xcd_block());
   }
 }
+
 
 template <class M, bool STRIDE>
 __global__ void __launch_bounds__(64) k_lu_wide(const ReactArgs a, const int* __restrict__ list,
@@ -3351,9 +3344,9 @@ hipError_t launch_lu(const ReactArgs& a, const int* list, const unsigned int* co
         hipLaunchKernelGGL((k_lu_wide<M, true>), g, dim3(64), 0, stream, a, list, count);
     } else {
       if (exact)
-        hipLaunchKernelGGL((k_lu_wide_reg<M, false>), g, dim3(64), 0, stream, a, list, count);
+        hipLaunchKernelGGL((k_lu_wide2<M, false>), g, dim3(128), 0, stream, a, list, count);
       else
-        hipLaunchKernelGGL((k_lu_wide_reg<M, true>), g, dim3(64), 0, stream, a, list, count);
+        hipLaunchKernelGGL((k_lu_wide2<M, true>), g, dim3(128), 0, stream, a, list, count);
     }
   } else {
   if (a.lu_mfma) {   // one cell per block; a multiple of 8 blocks (XCD mapping)
