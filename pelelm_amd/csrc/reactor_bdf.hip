@@ -27,6 +27,10 @@
 #include <cmath>
 #include <cstddef>
 #include <utility>
+#ifdef PELE_CHECK_BOUNDS
+#include <cstdio>
+#include <cstdlib>
+#endif
 
 #include "reactor_internal.h"
 #include "mlp_internal.h"
@@ -147,19 +151,53 @@ struct MatSlots {
   static constexpr int STRIDE = (USED + 15) / 16 * 16; // whole 128-byte lines per cell
 };
 
+// PELE_CHECK_BOUNDS (host builds of this file only, never a device build): every slot
+// index of a cell's workspace and matrix region, and every FP32 factor index, is
+// checked against that cell's extent, and a violation aborts with the index.
+#ifdef PELE_CHECK_BOUNDS
+inline void pele_check(bool ok, const char* what, long i, long lim) {
+  if (!ok) {
+    std::fprintf(stderr, "bounds: %s index %ld outside [0, %ld)\n", what, i, lim);
+    std::abort();
+  }
+}
+#define PELE_CHECK(ok, what, i, lim) pele_check((ok), (what), (long)(i), (long)(lim))
+#define PELE_LIM(x) , (x)
+#define PELE_LIM_FIELD int lim = 1 << 30;
+#else
+#define PELE_CHECK(ok, what, i, lim) ((void)0)
+#define PELE_LIM(x)
+#define PELE_LIM_FIELD
+#endif
+
 struct WS {
   double* p;
-  __device__ __forceinline__ double& operator[](int s) const { return p[(long)s * 64]; }
+  PELE_LIM_FIELD
+  __device__ __forceinline__ double& operator[](int s) const {
+    PELE_CHECK(s >= 0 && s < lim, "workspace slot", s, lim);
+    return p[(long)s * 64];
+  }
 };
 struct IWS {
   int* p;
-  __device__ __forceinline__ int& operator[](int s) const { return p[(long)s * 64]; }
+  PELE_LIM_FIELD
+  __device__ __forceinline__ int& operator[](int s) const {
+    PELE_CHECK(s >= 0 && s < lim, "int workspace slot", s, lim);
+    return p[(long)s * 64];
+  }
 };
 // one cell's matrix region
 struct CM {
   double* p;
-  __device__ __forceinline__ double& operator[](int s) const { return p[s]; }
-  __device__ __forceinline__ int& perm(int base, int i) const { return reinterpret_cast<int*>(p + base)[i]; }
+  PELE_LIM_FIELD
+  __device__ __forceinline__ double& operator[](int s) const {
+    PELE_CHECK(s >= 0 && s < lim, "matrix-region word", s, lim);
+    return p[s];
+  }
+  __device__ __forceinline__ int& perm(int base, int i) const {
+    PELE_CHECK(base >= 0 && i >= 0 && 2L * base + i < 2L * lim, "packed int", 2L * base + i, 2L * lim);
+    return reinterpret_cast<int*>(p + base)[i];
+  }
 };
 
 __device__ __forceinline__ long cell_off(const ArrView& v, int i, int j, int k) {
@@ -285,6 +323,7 @@ struct LUAcc<N1, double> {
 template <int N1>
 struct LUAcc<N1, float> {
   static __device__ __forceinline__ float& at(const CM& m, int e) {
+    PELE_CHECK(e >= 0 && e < N1 * N1, "FP32 factor", e, N1 * N1);
     return reinterpret_cast<float*>(&m[MatSlots<N1>::LU])[e];
   }
 };
@@ -1003,16 +1042,16 @@ struct Lane {
 
 template <class M>
 __device__ __forceinline__ WS cell_ws(double* ws, long c) {
-  return WS{ws + (c >> 6) * (long)Slots<M::N1>::TOTAL * 64 + (c & 63)};
+  return WS{ws + (c >> 6) * (long)Slots<M::N1>::TOTAL * 64 + (c & 63) PELE_LIM(Slots<M::N1>::TOTAL)};
 }
 template <class M>
 __device__ __forceinline__ IWS cell_iws(int* iws, long c) {
-  return IWS{iws + (c >> 6) * (long)Slots<M::N1>::ITOTAL * 64 + (c & 63)};
+  return IWS{iws + (c >> 6) * (long)Slots<M::N1>::ITOTAL * 64 + (c & 63) PELE_LIM(Slots<M::N1>::ITOTAL)};
 }
 
 template <class M>
 __device__ __forceinline__ CM cell_mat(double* mat, long c) {
-  return CM{mat + c * (long)MatSlots<M::N1>::STRIDE};
+  return CM{mat + c * (long)MatSlots<M::N1>::STRIDE PELE_LIM(MatSlots<M::N1>::STRIDE)};
 }
 
 // The lane state persists between the kernels of a trip as words of the workspace (the
