@@ -3066,7 +3066,9 @@ __global__ void __launch_bounds__(64) PELE_WAVES_PER_EU(PELE_EVAL_WAVES(M::N1)) 
 // pivoted (ties to the lowest row, a NaN as infinite: one 64-bit max over the wave and
 // a ballot, rows below 64 first), multipliers m_rk (1 / pivot) and the row updates from
 // the pivot row, read from LDS by every lane at once.  Rows stay in place; factor row s is
-// the row chosen at step s (PERM[s]), as the cooperative solves read them.  FP64 only.
+// the row chosen at step s (PERM[s]).  The factors are stored by columns (element (s, j)
+// at LU + j N1 + s), so the solve's lanes, one factor row each, read consecutive words of
+// a column.  FP64 only.
 // ---------------------------------------------------------------------------
 template <int N1>
 struct Wide {
@@ -3171,7 +3173,7 @@ __device__ __forceinline__ void lu_wide_one(const ReactArgs& a, const int* __res
   }
 #pragma unroll 1
   for (int e = l; e < N1 * N1; e += 64) {
-    const int st = e / N1, j = e - (e / N1) * N1;
+    const int j = e / N1, st = e - (e / N1) * N1;   // by columns: column j, factor row st
     mat[MS::LU + e] = sm[sstep[st] * LD + j];
   }
 #pragma unroll 1
@@ -3184,7 +3186,7 @@ __device__ __forceinline__ void lu_wide_one(const ReactArgs& a, const int* __res
 // t in registers, the column loop unrolled at compile time): each step's pivot from the
 // two waves' 64-bit maxima exchanged through LDS (ties to the lower row), the pivot row
 // broadcast through LDS, both double-buffered by step parity so one barrier pair per
-// column suffices.
+// column suffices.  Factor row s is written to column positions LU + j N1 + s.
 template <class M>
 __device__ __forceinline__ void lu_wide2_one(const ReactArgs& a, const int* __restrict__ list, const long n,
                                              const long idx) {
@@ -3254,7 +3256,7 @@ __device__ __forceinline__ void lu_wide2_one(const ReactArgs& a, const int* __re
   });
   if (row) {
 #pragma unroll
-    for (int j = 0; j < N1; ++j) mat[MS::LU + my_step * N1 + j] = m[j];
+    for (int j = 0; j < N1; ++j) mat[MS::LU + j * N1 + my_step] = m[j];   // by columns
     mat.perm(MS::PERM, my_step) = t;
   }
   if (t == 0) w[S::SING] = (double)sing;
@@ -3286,8 +3288,8 @@ __global__ void __launch_bounds__(64) k_lu_wide(const ReactArgs a, const int* __
 
 // The Newton solves for these cells: x = M^-1 B with the factors above, lane l holding
 // components l and l + 64; forward and back substitution column by column, each finished
-// component broadcast with v_readlane (factor rows read from the matrix region, a lane's
-// row walked in order through the cache).
+// component broadcast with v_readlane (each column of the factors read from the matrix
+// region as 64 consecutive words: one load covers a few whole lines, not a line per lane).
 template <class M>
 __device__ __forceinline__ void solve_wide_one(const ReactArgs& a, const int* __restrict__ list, const long n,
                                                const long idx) {
@@ -3303,17 +3305,18 @@ __device__ __forceinline__ void solve_wide_one(const ReactArgs& a, const int* __
   const bool has0 = r0 < N1, has1 = r1 < N1;
   double y0 = has0 ? mat[MS::B + mat.perm(MS::PERM, r0)] : 0.0;
   double y1 = has1 ? mat[MS::B + mat.perm(MS::PERM, r1)] : 0.0;
-  // eight columns of the lane's rows are loaded ahead of their eight dependent steps
-  const double* f0 = &mat[MS::LU + (has0 ? r0 : 0) * N1];
-  const double* f1 = &mat[MS::LU + (has1 ? r1 : 0) * N1];
+  // eight columns of the lane's rows are loaded ahead of their eight dependent steps; the
+  // factors are stored by columns, so a load reads 64 consecutive words
+  const double* f0 = &mat[MS::LU + (has0 ? r0 : 0)];
+  const double* f1 = &mat[MS::LU + (has1 ? r1 : 0)];
 #pragma unroll 1
   for (int k0 = 0; k0 < N1 - 1; k0 += 8) {
     double c0[8], c1[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int k = k0 + t;
-      c0[t] = (has0 && r0 > k && k < N1 - 1) ? f0[k] : 0.0;
-      c1[t] = (has1 && r1 > k && k < N1 - 1) ? f1[k] : 0.0;
+      c0[t] = (has0 && r0 > k && k < N1 - 1) ? f0[k * N1] : 0.0;
+      c1[t] = (has1 && r1 > k && k < N1 - 1) ? f1[k * N1] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -3331,8 +3334,8 @@ __device__ __forceinline__ void solve_wide_one(const ReactArgs& a, const int* __
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int k = k0 - t;   // the diagonal for the owner, U entries for the rows above
-      c0[t] = (has0 && r0 <= k && k >= 0) ? f0[k] : 0.0;
-      c1[t] = (has1 && r1 <= k && k >= 0) ? f1[k] : 0.0;
+      c0[t] = (has0 && r0 <= k && k >= 0) ? f0[k * N1] : 0.0;
+      c1[t] = (has1 && r1 <= k && k >= 0) ? f1[k * N1] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
