@@ -1478,6 +1478,37 @@ __device__ __forceinline__ double lane_read(const double x, const int src) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// The largest 64-bit key over each group of G lanes (32: a half-wave, 64: the wave), in
+// every lane of the group: butterfly maxima within each 16-lane row through DPP
+// (quad_perm [1,0,3,2] and [2,3,0,1], then the half-row and row mirrors), then the four
+// rows' maxima read with v_readlane.  It stays in the vector and scalar units, where a
+// __shfl_xor ladder sends each of its levels through the LDS crossbar.  Every lane of
+// the wave must take part.
+template <int G>
+__device__ __forceinline__ unsigned long long group_max_u64(unsigned long long v) {
+  static_assert(G == 32 || G == 64, "a half-wave or the whole wave");
+  auto step = [&](auto ctrl) {
+    constexpr int C = decltype(ctrl)::value;
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)v, C, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), C, 0xF, 0xF, false);
+    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+    v = o > v ? o : v;
+  };
+  step(std::integral_constant<int, 0xB1>{});    // quad_perm [1,0,3,2]: lane ^ 1
+  step(std::integral_constant<int, 0x4E>{});    // quad_perm [2,3,0,1]: lane ^ 2
+  step(std::integral_constant<int, 0x141>{});   // row_half_mirror: the other quad of the half-row
+  step(std::integral_constant<int, 0x140>{});   // row_mirror: the other half of the row
+  auto row_max = [&](int lane) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
+    return ((unsigned long long)hi << 32) | lo;
+  };
+  const unsigned long long r0 = row_max(0), r1 = row_max(16), r2 = row_max(32), r3 = row_max(48);
+  const unsigned long long a = r0 > r1 ? r0 : r1, b = r2 > r3 ? r2 : r3;
+  if (G == 64) return a > b ? a : b;
+  return (threadIdx.x & 32) ? b : a;
+}
+
 // x as held by lane `src` (a compile-time constant) of this lane's group of G lanes:
 // v_readlane for a whole-wave group (measured faster than a permute); two groups per
 // wave would need both lanes read and a select, which measured no faster than the
@@ -1541,12 +1572,7 @@ __device__ __forceinline__ int lu_coop_core(const CM& mat, const bool work, cons
       if (av != av) av = HUGE_VAL;
       key = (unsigned long long)__double_as_longlong(av) + 1ull;
     }
-    unsigned long long kmax = key;
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-      const unsigned long long o = __shfl_xor(kmax, off, G);
-      kmax = o > kmax ? o : kmax;
-    }
+    const unsigned long long kmax = group_max_u64<G>(key);
     const unsigned long long ball = __ballot(key == kmax);
     const unsigned long long mine = (G == 64) ? ball : ((ball >> (half * 32)) & 0xFFFFFFFFull);
     const int who = __ffsll((long long)mine) - 1;
@@ -1738,12 +1764,7 @@ __device__ __forceinline__ int lu_mfma_core(const CM& mat, const bool work, cons
           if (av != av) av = HUGE_VAL;
           key = (unsigned long long)__double_as_longlong(av) + 1ull;
         }
-        unsigned long long kmax = key;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const unsigned long long o = __shfl_xor(kmax, off, 64);
-          kmax = o > kmax ? o : kmax;
-        }
+        const unsigned long long kmax = group_max_u64<64>(key);
         const int who = __ffsll((long long)__ballot(key == kmax)) - 1;
         const bool nonzero = kmax > 1ull;
         if (!nonzero && sing == 0) sing = k + 1;
@@ -3110,12 +3131,7 @@ __device__ __forceinline__ void lu_wide_one(const ReactArgs& a, const int* __res
       return (unsigned long long)__double_as_longlong(av) + 1ull;
     };
     const unsigned long long k0 = key_of(r0, done0), k1 = key_of(r1, done1);
-    unsigned long long kmax = k0 > k1 ? k0 : k1;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const unsigned long long o = __shfl_xor(kmax, off, 64);
-      kmax = o > kmax ? o : kmax;
-    }
+    const unsigned long long kmax = group_max_u64<64>(k0 > k1 ? k0 : k1);
     const unsigned long long b0 = __ballot(k0 == kmax);
     const unsigned long long b1 = __ballot(k1 == kmax);
     const int prow = b0 ? __ffsll((long long)b0) - 1 : 64 + __ffsll((long long)b1) - 1;
@@ -3219,12 +3235,7 @@ __device__ __forceinline__ void lu_wide2_one(const ReactArgs& a, const int* __re
       if (av != av) av = HUGE_VAL;
       key = (unsigned long long)__double_as_longlong(av) + 1ull;
     }
-    unsigned long long kmax = key;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const unsigned long long o = __shfl_xor(kmax, off, 64);
-      kmax = o > kmax ? o : kmax;
-    }
+    const unsigned long long kmax = group_max_u64<64>(key);
     const unsigned long long ball = __ballot(key == kmax);
     if ((t & 63) == 0) {
       wkey[par][wave] = kmax;

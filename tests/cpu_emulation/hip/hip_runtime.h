@@ -77,6 +77,8 @@ inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline long long __double_as_longlong(double x) { long long r; __builtin_memcpy(&r, &x, 8); return r; }
 inline double __longlong_as_double(long long x) { double r; __builtin_memcpy(&r, &x, 8); return r; }
 #define __builtin_amdgcn_readlane(v, l) (v)
+// one lane at a time: a DPP move reads this lane's own value
+#define __builtin_amdgcn_update_dpp(old, src, ctrl, rm, bm, bc) (src)
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline unsigned int __float_as_uint(float x) {
   unsigned int u;

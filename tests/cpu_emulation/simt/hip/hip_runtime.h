@@ -319,6 +319,25 @@ inline int simt_readlane(int v, int l) {
   return simt_val<int>(x[base + l]);
 }
 #define __builtin_amdgcn_readlane(v, l) simt_readlane((v), (l))
+// DPP source lanes for the controls the kernels use (quad_perm, row_mirror,
+// row_half_mirror; every row and bank enabled, every lane of the wave taking part)
+inline int simt_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
+  (void)old;
+  (void)bound_ctrl;
+  if (row_mask != 0xF || bank_mask != 0xF) simt::die("dpp masks not emulated");
+  const int me = simt::W->cur, lane = me % simt::WAVE, base = me - lane;
+  const uint64_t* x = simt::exchange(simt_bits(src), 7);
+  const int row = lane & ~15, i = lane & 15;
+  int s;
+  if (ctrl < 0x100) s = (lane & ~3) + ((ctrl >> (2 * (lane & 3))) & 3);
+  else if (ctrl == 0x140) s = row + 15 - i;
+  else if (ctrl == 0x141) s = row + (i & 8) + 7 - (i & 7);
+  else simt::die("dpp control not emulated");
+  if (base + s >= simt::W->nlanes || !simt::took_part(base + s, me)) simt::die("dpp from a lane that has returned");
+  return simt_val<int>(x[base + s]);
+}
+#define __builtin_amdgcn_update_dpp(old, src, ctrl, rm, bm, bc) \
+  simt_update_dpp((old), (src), (ctrl), (rm), (bm), (bc))
 // the learned predictor's matrix-core products are not emulated (its kernels are not
 // launched here)
 #define __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, x, y, z) (c)

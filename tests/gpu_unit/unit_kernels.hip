@@ -349,6 +349,14 @@ __global__ void __launch_bounds__(64) k_mfma_probe(const double* a, const double
   for (int v = 0; v < 4; ++v) d[4 * l + v] = (double)cv[v];
 }
 
+// the pivot searches' group maximum on raw lane keys: lane l passes key[l] and returns
+// the maximum over its half-wave (out[l]) and over the wave (out[64 + l])
+__global__ void __launch_bounds__(64) k_group_max(const unsigned long long* key, unsigned long long* out) {
+  const int l = threadIdx.x;
+  out[l] = group_max_u64<32>(key[l]);
+  out[64 + l] = group_max_u64<64>(key[l]);
+}
+
 // one v_mfma_f32_16x16x32_bf16 on raw lane registers: lane l passes a[8 l + i], b[8 l + i]
 // (i = 0..7, values exactly representable in bf16) and c[4 l + v], and returns d[4 l + v]
 typedef __bf16 unit_bf16x8 __attribute__((ext_vector_type(8)));
@@ -474,6 +482,22 @@ extern "C" int unit_mfma_probe(int f64, const double* A, const double* B, const 
   UNIT_SYNC();
   UNIT_D2H(D, dD, 256 * sizeof(double));
   for (double* p : {dA, dB, dC, dD}) UNIT_DEV_FREE(p);
+  return e == hipSuccess ? 0 : -3;
+#endif
+}
+
+// key: 64 lane keys; out: 128 (half-wave maxima, then wave maxima)
+extern "C" int unit_group_max(const unsigned long long* key, unsigned long long* out) {
+#ifdef UNIT_HOST
+  (void)key; (void)out;
+  return -2;   // a DPP maximum needs a whole wave
+#else
+  unsigned long long *dk = dev_copy(key, 64), *dout = dev_zero<unsigned long long>(128);
+  hipLaunchKernelGGL(k_group_max, dim3(1), dim3(64), 0, nullptr, dk, dout);
+  const hipError_t e = hipGetLastError();
+  UNIT_SYNC();
+  UNIT_D2H(out, dout, 128 * sizeof(unsigned long long));
+  for (unsigned long long* p : {dk, dout}) UNIT_DEV_FREE(p);
   return e == hipSuccess ? 0 : -3;
 #endif
 }

@@ -390,6 +390,34 @@ def test_matrix_core_operand_layout(f64):
 
 
 @pytest.mark.gpu
+def test_dpp_group_maximum_of_pivot_keys():
+    """The pivot searches' 64-bit maximum through DPP moves and v_readlane, on raw lane
+    keys: every lane must return its half-wave's and the wave's maximum exactly.  The
+    keys mix random 64-bit values (high words decide), keys equal in the high word (the
+    low word decides), ties, zeros and a maximum in each of the four 16-lane rows."""
+    G = _load("libunit_gpu.so")
+    rng = np.random.default_rng(11)
+    cases = [rng.integers(0, 2**63, size=64, dtype=np.uint64) for _ in range(3)]
+    hi = np.uint64(0x3FF0000000000000)
+    cases.append(hi + rng.integers(0, 2**20, size=64, dtype=np.uint64))   # equal high words
+    cases.append(np.zeros(64, dtype=np.uint64))
+    tie = np.ones(64, dtype=np.uint64)
+    tie[[5, 40]] = np.uint64(2**40 + 1)
+    cases.append(tie)
+    for row in range(4):
+        k = rng.integers(0, 2**32, size=64, dtype=np.uint64)
+        k[16 * row + 13] = np.uint64(2**62 + 7)
+        cases.append(k)
+    for key in cases:
+        key = np.ascontiguousarray(key, dtype=np.uint64)
+        out = np.zeros(128, dtype=np.uint64)
+        assert G.unit_group_max(_p(key), _p(out)) == 0
+        halves = np.repeat([key[:32].max(), key[32:].max()], 32)
+        np.testing.assert_array_equal(out[:64], halves)
+        np.testing.assert_array_equal(out[64:], np.full(64, key.max(), dtype=np.uint64))
+
+
+@pytest.mark.gpu
 def test_bf16_matrix_core_operand_layout():
     """One v_mfma_f32_16x16x32_bf16 (the learned predictor's product) on raw lane
     registers with the layout the network kernels assume: lane l holds A[l % 16][8 (l / 16)
