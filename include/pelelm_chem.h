@@ -109,8 +109,9 @@ int mi_chem_init(void* reactor, int reactor_type, int ncells);
      of the right-hand side, no set-up; the trip schedules only, mi.gmres_maxl Krylov
      vectors, default 5 as CVODE's, at most the number of species), peleLM.use_typ_vals_chem (0/1),
      mi.mechanism (drm19 | LiDryer | dodecane53 | dodecane56 | heptane88; above 63 species the
-     factorisations and solves take one wave per cell with two matrix rows per lane, in FP64,
-     and the persistent schedule and the learned predictor are not available), mi.schedule (plain: fixed cell-to-lane mapping
+     factorisations take two waves per cell with one matrix row per thread and the solves
+     one wave per cell with two rows per lane, in FP64, and the persistent schedule and the
+     learned predictor are not available), mi.schedule (plain: fixed cell-to-lane mapping
      for the evaluation and step kernels, default | queue: finished cells leave the work
      lists | auto: the first call of each box size searches the plain and queue schedules
      (host-driven or in graphs of 8 trips, with cost ordering, plain also without) and the
